@@ -1,0 +1,294 @@
+"""adeco_amd — MI355X-native exact phase-space-tetrahedron mass deposit (ADECO's hot path).
+
+Python host mirror of the reference's C++ call surface, over the C ABI of libadeco_amd.so
+(include/adeco.h).  Names, argument meaning and error behaviour follow the reference:
+
+* ``DepositWrapper.deposit_tets`` — AdaptiveMassDeposit::DepositWrapper::deposit_tets
+  (AdDeposit2Grid.h:82-91): deposits meta-patches into a float grid, accumulating (``+=``), and
+  raises ``AdRuntimeException`` where the reference throws.
+* ``ResidentLattice`` — the lattice kept in HBM for repeated deposits (octree leaves, ROI requests).
+
+All compute runs in HIP kernels on the GPU; the library fails loudly when it or the GPU is missing.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from ._lib import load, ptr
+
+ADECO_OK = 0
+ADECO_EINVAL = 1
+ADECO_EOVERFLOW = 2
+ADECO_ENOMEM = 3
+ADECO_EDEVICE = 4
+ADECO_EINTERNAL = 5
+
+DECOMP_REF_EFFECTIVE = 0
+DECOMP_REF_INTENDED = 1
+DECOMP_KUHN = 2
+
+MODE_PLAIN = 0
+MODE_BATCHED = 1
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+PREC_FP32 = 2
+
+NUM_STATS = 11
+STAT_NAMES = ("tets", "tets_active", "candidates", "aabb_pass", "none", "tet_in_cell", "cell_in_tet",
+              "partial", "nonzero", "clip_stages", "clip_subtets")
+
+
+class AdRuntimeException(RuntimeError):
+    """Raised where the reference throws AdaptiveMassDeposit::AdRuntimeException (AdException.h:87-95)."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+def _check(status: int, what: str = ""):
+    if status != ADECO_OK:
+        msg = load().adeco_last_error().decode(errors="replace")
+        raise AdRuntimeException(status, f"ERROR: {what}: {msg}" if what else f"ERROR: {msg}")
+
+
+def device_count() -> int:
+    return load().adeco_device_count()
+
+
+def last_kernel_ms() -> float:
+    """HIP-event time of the calling thread's last deposit launch(es), in ms."""
+    return load().adeco_last_kernel_ms()
+
+
+def set_memory_budget(nbytes: int):
+    """Per-call device memory budget (0 = 75% of device memory); lower values force multi-pass."""
+    _check(load().adeco_set_memory_budget(int(nbytes)), "set_memory_budget()")
+
+
+class DepositWrapper:
+    """Mirror of AdaptiveMassDeposit::DepositWrapper (AdDeposit2Grid.h:59-93)."""
+
+    h_grid_mass_type = np.float32
+    vertex_type = np.float32
+    mass_type = np.float32
+
+    @staticmethod
+    def get_cuda_grid_size() -> int:
+        return load().adeco_get_grid_size()
+
+    @staticmethod
+    def get_cuda_block_size() -> int:
+        return load().adeco_get_block_size()
+
+    @staticmethod
+    def deposit_tets(use_specific_gpu: bool, device_id_to_use: int, tet_offset: int, num_tets_2_process: int,
+                     positions: np.ndarray, positions_size: int, meta_cells_per_patch: int, bbox,
+                     dims: int, data: np.ndarray) -> None:
+        """Deposits patches [tet_offset, tet_offset + num_tets_2_process) into ``data`` (+=).
+
+        As in the reference, the two size arguments count meta-patches, not tets
+        (AdDeposit2Grid.cpp:69-70).  ``positions`` is float32 AoS meta-patch data, ``bbox`` is
+        x0,x1,y0,y1,z0,z1 and ``data`` a float32 array of dims^3 cells (i fastest), updated in place.
+        """
+        positions = np.ascontiguousarray(positions, dtype=np.float32)
+        bbox = np.ascontiguousarray(bbox, dtype=np.float32)
+        if data.dtype != np.float32 or not data.flags.c_contiguous:
+            raise AdRuntimeException(ADECO_EINVAL, "ERROR: deposit_tets(): data must be a contiguous float32 array")
+        if data.size != int(dims) ** 3 or bbox.size != 6:
+            raise AdRuntimeException(ADECO_EINVAL, "ERROR: deposit_tets(): inconsistent grid or bbox size")
+        if int(positions_size) > positions.size:
+            raise AdRuntimeException(ADECO_EINVAL, "ERROR: deposit_tets(): positions_size exceeds the positions array")
+        st = load().adeco_deposit_patches(int(bool(use_specific_gpu)), int(device_id_to_use), int(tet_offset),
+                                          int(num_tets_2_process), ptr(positions), int(positions_size),
+                                          int(meta_cells_per_patch), ptr(bbox), int(dims), ptr(data))
+        _check(st, "deposit_tets()")
+
+
+class ResidentLattice:
+    """A vertex lattice uploaded once to one GPU (SoA float32, i fastest)."""
+
+    def __init__(self, x, y, z, dims, device: int = 0):
+        lib = load()
+        self._x = np.ascontiguousarray(x, dtype=np.float32)
+        self._y = np.ascontiguousarray(y, dtype=np.float32)
+        self._z = np.ascontiguousarray(z, dtype=np.float32)
+        self.dims = tuple(int(d) for d in dims)
+        self.device = device
+        h = _lib.c_vp()
+        _check(lib.adeco_lattice_create(device, ptr(self._x), ptr(self._y), ptr(self._z), *self.dims,
+                                        _lib.ctypes.byref(h)), "lattice_create()")
+        self.handle = h
+
+    @classmethod
+    def from_lattice(cls, lat, device: int = 0):
+        return cls(lat.x, lat.y, lat.z, lat.dims, device)
+
+    @property
+    def num_tets(self) -> int:
+        return 6 * (self.dims[0] - 1) * (self.dims[1] - 1) * (self.dims[2] - 1)
+
+    def bbox(self) -> np.ndarray:
+        out = np.zeros(6, np.float32)
+        _check(load().adeco_lattice_bbox(self.handle, ptr(out)), "lattice_bbox()")
+        return out
+
+    def deposit_grid(self, bbox, dims: int, decomposition: int = DECOMP_REF_EFFECTIVE, tet_mass: float = 1.0,
+                     tet_ids=None, mode: int = MODE_PLAIN, precision: int = PREC_FP64, stats: bool = False):
+        """Deposits all (or the listed) lattice tets onto one dims^3 FP64 grid; returns it (and stats)."""
+        bbox = np.ascontiguousarray(bbox, dtype=np.float32)
+        grid = np.zeros(int(dims) ** 3, np.float64)
+        ids = None if tet_ids is None else np.ascontiguousarray(tet_ids, dtype=np.int32)
+        st = np.zeros(NUM_STATS, np.uint64) if stats else None
+        _check(load().adeco_lattice_deposit_grid(self.handle, int(decomposition), float(tet_mass), ptr(ids),
+                                                 0 if ids is None else ids.size, ptr(bbox), int(dims), int(mode),
+                                                 int(precision), ptr(grid), ptr(st)), "lattice_deposit_grid()")
+        return (grid, st) if stats else grid
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            load().adeco_lattice_destroy(self.handle)
+            self.handle = _lib.c_vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+
+class Octree:
+    """The reference's one-rank octree over a resident lattice (MetaMesh + generate_tree + Octree).
+
+    Mirrors AMD_Parameters' octree keys: ``max_tets_per_octree_node``, ``max_refinement_level``
+    and the optional ``region_of_interest`` (x0,x1,y0,y1,z0,z1).  Node ids, counts, the DFS leaf
+    list and the leaf processing order are the reference's.
+    """
+
+    def __init__(self, lattice: ResidentLattice, max_tets_per_octree_node: int, max_refinement_level: int,
+                 region_of_interest=None):
+        self.lattice = lattice
+        roi = None if region_of_interest is None else np.ascontiguousarray(region_of_interest, np.float32)
+        h = _lib.c_vp()
+        _check(load().adeco_octree_build(lattice.handle, ptr(roi), int(max_tets_per_octree_node),
+                                         int(max_refinement_level), _lib.ctypes.byref(h)), "octree_build()")
+        self.handle = h
+        self.dims = None
+
+    @property
+    def num_nodes(self) -> int:
+        return load().adeco_octree_num_nodes(self.handle)
+
+    @property
+    def num_leaves(self) -> int:
+        return load().adeco_octree_num_leaves(self.handle)
+
+    @property
+    def num_meta_cells(self) -> int:
+        return load().adeco_octree_num_meta_cells(self.handle)
+
+    def bbox(self) -> np.ndarray:
+        out = np.zeros(6, np.float32)
+        _check(load().adeco_octree_bbox(self.handle, ptr(out)), "octree_bbox()")
+        return out
+
+    def nodes(self) -> dict:
+        n = self.num_nodes
+        d = {"ll": np.zeros((n, 3), np.float32), "ext": np.zeros(n, np.float32), "level": np.zeros(n, np.int32),
+             "parent": np.zeros(n, np.int32), "first_child": np.zeros(n, np.int32), "count": np.zeros(n, np.uint64)}
+        _check(load().adeco_octree_nodes(self.handle, ptr(d["ll"]), ptr(d["ext"]), ptr(d["level"]), ptr(d["parent"]),
+                                         ptr(d["first_child"]), ptr(d["count"])), "octree_nodes()")
+        return d
+
+    def leaves(self) -> dict:
+        lib = load()
+        nl = self.num_leaves
+        total = lib.adeco_octree_leaves(self.handle, None, None, None, None)
+        d = {"dfs": np.zeros(nl, np.int32), "order": np.zeros(nl, np.int32),
+             "mc_offsets": np.zeros(nl + 1, np.int64), "mc_ids": np.zeros(max(total, 1), np.int32)}
+        lib.adeco_octree_leaves(self.handle, ptr(d["dfs"]), ptr(d["order"]), ptr(d["mc_offsets"]), ptr(d["mc_ids"]))
+        d["mc_ids"] = d["mc_ids"][:total]
+        return d
+
+    def meta_bboxes(self) -> np.ndarray:
+        out = np.zeros((self.num_meta_cells, 6), np.float32)
+        _check(load().adeco_octree_meta_bboxes(self.handle, ptr(out)), "octree_meta_bboxes()")
+        return out
+
+    def leaf_boxes(self) -> np.ndarray:
+        """Float (ll, ll + ext) boxes of the leaves in DFS order, as the reference forms them."""
+        nd = self.nodes()
+        dfs = self.leaves()["dfs"]
+        out = np.empty((dfs.size, 6), np.float32)
+        out[:, 0::2] = nd["ll"][dfs]
+        out[:, 1::2] = (nd["ll"][dfs] + nd["ext"][dfs][:, None]).astype(np.float32)
+        return out
+
+    def bin(self, decomposition: int = DECOMP_REF_EFFECTIVE) -> np.ndarray:
+        counts = np.zeros(self.num_leaves, np.int64)
+        _check(load().adeco_octree_bin(self.handle, int(decomposition), ptr(counts)), "octree_bin()")
+        return counts
+
+    def deposit(self, dims: int, decomposition: int = DECOMP_REF_EFFECTIVE, tet_mass: float = 1.0,
+                mode: int = MODE_PLAIN, precision: int = PREC_FP64) -> float:
+        """deposit_leaf_nodes on one rank; returns the kernel time in ms (grids stay on the GPU)."""
+        _check(load().adeco_deposit_leaves(self.handle, int(dims), int(decomposition), float(tet_mass), int(mode),
+                                           int(precision)), "deposit_leaves()")
+        self.dims = int(dims)
+        return last_kernel_ms()
+
+    def leaf_grids(self, first: int = 0, count=None) -> np.ndarray:
+        """FP64 mass grids [count, dims^3] of leaves first..first+count (DFS order)."""
+        count = self.num_leaves - first if count is None else count
+        out = np.zeros((count, self.dims ** 3), np.float64)
+        _check(load().adeco_leaf_grids(self.handle, int(first), int(count), ptr(out)), "leaf_grids()")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            load().adeco_octree_destroy(self.handle)
+            self.handle = _lib.c_vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def deposit_soup(tets, bbox, dims: int, tet_mass: float = 1.0, device: int = 0, mode: int = MODE_PLAIN,
+                 precision: int = PREC_FP64, stats: bool = False):
+    """Deposits a float32 tet soup [n, 4, 3] onto one dims^3 FP64 grid."""
+    tets = np.ascontiguousarray(tets, dtype=np.float32).reshape(-1, 12)
+    bbox = np.ascontiguousarray(bbox, dtype=np.float32)
+    grid = np.zeros(int(dims) ** 3, np.float64)
+    st = np.zeros(NUM_STATS, np.uint64) if stats else None
+    _check(load().adeco_deposit_soup(device, ptr(tets), tets.shape[0], ptr(bbox), int(dims), float(tet_mass),
+                                     int(mode), int(precision), ptr(grid), ptr(st)), "deposit_soup()")
+    return (grid, st) if stats else grid
+
+
+def cubic_root_box(bbox) -> np.ndarray:
+    """The octree root box of a bbox: (min, min + max extent) per axis in float32 (AdOctreeGeneration.h:98)."""
+    b = np.asarray(bbox, dtype=np.float32)
+    ext = max(b[1] - b[0], b[3] - b[2], b[5] - b[4])
+    mn = b[0::2]
+    out = np.empty(6, np.float32)
+    out[0::2] = mn
+    out[1::2] = (mn + np.float32(ext)).astype(np.float32)
+    return out

@@ -1,0 +1,85 @@
+"""ctypes loader for libadeco_amd.so (built in-tree by adeco_amd/build.py).
+
+The library must exist: there is no fallback.  Every compute entry point runs HIP kernels and
+returns ADECO_EDEVICE when no GPU is usable.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libadeco_amd.so")
+
+c_f32p = ctypes.POINTER(ctypes.c_float)
+c_f64p = ctypes.POINTER(ctypes.c_double)
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_u64p = ctypes.POINTER(ctypes.c_uint64)
+c_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); mirrors include/adeco.h
+SIGNATURES = {
+    "adeco_last_error": (ctypes.c_char_p, []),
+    "adeco_version": (ctypes.c_int, []),
+    "adeco_device_count": (ctypes.c_int, []),
+    "adeco_deposit_patches": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                             c_vp, ctypes.c_size_t, ctypes.c_int, c_vp, ctypes.c_int, c_vp]),
+    "adeco_get_grid_size": (ctypes.c_int, []),
+    "adeco_get_block_size": (ctypes.c_int, []),
+    "adeco_set_memory_budget": (ctypes.c_int, [ctypes.c_size_t]),
+    "adeco_lattice_create": (ctypes.c_int, [ctypes.c_int, c_vp, c_vp, c_vp, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "adeco_lattice_destroy": (ctypes.c_int, [c_vp]),
+    "adeco_lattice_bbox": (ctypes.c_int, [c_vp, c_vp]),
+    "adeco_lattice_deposit_grid": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_double, c_vp, ctypes.c_int64,
+                                                  c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp, c_vp]),
+    "adeco_deposit_soup": (ctypes.c_int, [ctypes.c_int, c_vp, ctypes.c_int64, c_vp, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_int, ctypes.c_int, c_vp, c_vp]),
+    "adeco_last_kernel_ms": (ctypes.c_double, []),
+    "adeco_octree_build": (ctypes.c_int, [c_vp, c_vp, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "adeco_octree_destroy": (ctypes.c_int, [c_vp]),
+    "adeco_octree_num_nodes": (ctypes.c_int64, [c_vp]),
+    "adeco_octree_num_leaves": (ctypes.c_int64, [c_vp]),
+    "adeco_octree_num_meta_cells": (ctypes.c_int64, [c_vp]),
+    "adeco_octree_bbox": (ctypes.c_int, [c_vp, c_vp]),
+    "adeco_octree_nodes": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "adeco_octree_leaves": (ctypes.c_int64, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "adeco_octree_meta_bboxes": (ctypes.c_int, [c_vp, c_vp]),
+    "adeco_octree_bin": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp]),
+    "adeco_deposit_leaves": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_int]),
+    "adeco_leaf_grids": (ctypes.c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_vp]),
+    "adeco_selftest_clip_vol6_f64": (ctypes.c_double, [c_vp]),
+    "adeco_selftest_clip_vol6_f32": (ctypes.c_float, [c_vp]),
+    "adeco_selftest_classify_f64": (ctypes.c_int, [c_vp]),
+    "adeco_selftest_beyond_fraction_f64": (ctypes.c_double, [c_vp]),
+    "adeco_selftest_decomposition": (None, [ctypes.c_int, c_vp]),
+    "adeco_selftest_deposit_soup_f64": (ctypes.c_int, [c_vp, ctypes.c_int64, c_vp, ctypes.c_int, ctypes.c_double,
+                                                       c_vp]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads the library (once) and declares the C-ABI signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python adeco_amd/build.py` "
+                          "(there is no CPU fallback for the deposit)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a):
+    """Data pointer of a numpy array (or None)."""
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)

@@ -1,0 +1,353 @@
+// Exact tetrahedron ∩ cell geometry for the ADECO mass deposit, written for CDNA4 (gfx950) and
+// also compiled for the host so the CPU test suite can check the same code.
+//
+// Semantics follow the reference CPU deposit (AdDepositTets2Grid.h:509-665,
+// AdTetIntersection.h:183-476): per tet, the covered cell range of its AABB, the strict AABB cull,
+// the scaled cell-local tet s = (v - cell_min) / dx, a classification into
+// no-intersection / tet-inside-cell / cell-inside-tet / partial, and for partial cells the exact
+// volume |s ∩ [0,1]^3|.  The clip itself is this project's own formulation (the reference walks a
+// 16-case lookup table through six faces with 10-point scratch arrays per level, which on a GPU
+// means dynamically indexed private memory):
+//   * the x- and y-faces are applied only when they actually cut the tet; each cut turns a tet
+//     into one tet (1 vertex inside), or a triangular prism (2 or 3 inside) written as a 6-point
+//     sequence whose 3 sub-tets are the 4 consecutive points starting at 0, 1 and 2, so every
+//     sub-tet is addressed with selects, never a table lookup;
+//   * the z-slab of each remaining sub-tet is integrated in closed form: the fraction of a tet
+//     beyond a plane, from the signed distances d_i of its vertices, is
+//       1 vertex beyond (a = d > 0, others b_j >= 0 on the near side):  a^3 / prod_j (a + b_j)
+//       2 beyond (a, b) and 2 near (c, e):
+//         [a^2 b^2 + ab(a+b)(c+e) + ce(a^2+ab+b^2)] / ((a+c)(a+e)(b+c)(b+e))
+//       3 beyond: 1 - (the 1-vertex formula seen from the near vertex)
+//     (all terms positive: no cancellation), so a sub-tet costs one determinant and a few
+//     multiplies instead of two more table-driven clips.
+#pragma once
+
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define ADECO_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define ADECO_HD inline
+#endif
+
+namespace adeco {
+
+// Cube-corner codes (dx | dy<<1 | dz<<2) of the 4 vertices of each of the 6 tets of a lattice
+// cube.  REF_EFFECTIVE is the decomposition the reference CPU/GPU deposit actually applies: the
+// 6-tet table of AdDepositTets2Grid.h:213-218 over the vertex slots as sample_cell_vertices fills
+// them (AdDepositTets2Grid.h:947-972).  REF_INTENDED is the same table over the vert[] slot order
+// (AdDepositTets2Grid.h:188-192), which tiles the cube.  KUHN is the six tets around the
+// (0,0,0)-(1,1,1) diagonal.
+enum Decomposition : int { REF_EFFECTIVE = 0, REF_INTENDED = 1, KUHN = 2 };
+
+struct CornerTable {
+    unsigned char c[6][4];
+};
+
+ADECO_HD CornerTable corner_table(int decomp) {
+    CornerTable t;
+    // slot -> corner code for the two reference slot orders
+    const unsigned char eff[8] = {0, 1, 2, 3, 6, 7, 4, 5};
+    const unsigned char intended[8] = {0, 4, 2, 6, 1, 5, 3, 7};
+    const unsigned char conn[6][4] = {{1, 0, 2, 4}, {4, 2, 1, 3}, {3, 5, 1, 4},
+                                      {4, 5, 6, 3}, {3, 2, 6, 4}, {3, 7, 5, 6}};
+    const unsigned char perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    for (int i = 0; i < 6; ++i) {
+        if (decomp == KUHN) {
+            t.c[i][0] = 0;
+            t.c[i][1] = (unsigned char)(1 << perm[i][0]);
+            t.c[i][2] = (unsigned char)((1 << perm[i][0]) | (1 << perm[i][1]));
+            t.c[i][3] = 7;
+        } else {
+            const unsigned char* s = decomp == REF_INTENDED ? intended : eff;
+            for (int v = 0; v < 4; ++v) t.c[i][v] = s[conn[i][v]];
+        }
+    }
+    return t;
+}
+
+template <class R>
+struct V3 {
+    R x, y, z;
+};
+
+template <class R>
+ADECO_HD V3<R> mk(R x, R y, R z) {
+    V3<R> v;
+    v.x = x; v.y = y; v.z = z;
+    return v;
+}
+
+template <class R>
+ADECO_HD R comp(const V3<R>& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+template <class R>
+ADECO_HD void set_comp(V3<R>& v, int a, R val) {
+    if (a == 0) v.x = val;
+    else if (a == 1) v.y = val;
+    else v.z = val;
+}
+
+template <class R>
+ADECO_HD V3<R> sel(bool p, const V3<R>& a, const V3<R>& b) {
+    return mk(p ? a.x : b.x, p ? a.y : b.y, p ? a.z : b.z);
+}
+
+// 6 x signed volume in the reference's sign convention: (a-d) . ((b-d) x (c-d))
+// (AdTetIntersection.h:183-203)
+template <class R>
+ADECO_HD R vol6(const V3<R>& a, const V3<R>& b, const V3<R>& c, const V3<R>& d) {
+    const R ady = a.y - d.y, bdy = b.y - d.y, cdy = c.y - d.y;
+    const R adz = a.z - d.z, bdz = b.z - d.z, cdz = c.z - d.z;
+    return (a.x - d.x) * (bdy * cdz - bdz * cdy) + (b.x - d.x) * (cdy * adz - cdz * ady) +
+           (c.x - d.x) * (ady * bdz - adz * bdy);
+}
+
+template <class R>
+ADECO_HD R absr(R x) { return x < R(0) ? -x : x; }
+
+// Fraction of a tet's volume on the far side (d > 0) of a plane, from the 4 vertex distances.
+template <class R>
+ADECO_HD R beyond_fraction(R d0, R d1, R d2, R d3) {
+    const int n = int(d0 > R(0)) + int(d1 > R(0)) + int(d2 > R(0)) + int(d3 > R(0));
+    if (n == 0) return R(0);
+    if (n == 4) return R(1);
+    // sort so the beyond vertices come first (values only; order within groups is irrelevant)
+    R a0 = d0, a1 = d1, a2 = d2, a3 = d3;
+#define ADECO_SWAP_IF(p, q) \
+    if (!(p > R(0)) && (q > R(0))) { const R t_ = p; p = q; q = t_; }
+    ADECO_SWAP_IF(a0, a1) ADECO_SWAP_IF(a1, a2) ADECO_SWAP_IF(a2, a3)
+    ADECO_SWAP_IF(a0, a1) ADECO_SWAP_IF(a1, a2) ADECO_SWAP_IF(a0, a1)
+#undef ADECO_SWAP_IF
+    if (n == 1) {
+        const R a = a0, b1 = -a1, b2 = -a2, b3 = -a3;
+        return (a * a * a) / ((a + b1) * (a + b2) * (a + b3));
+    }
+    if (n == 2) {
+        const R a = a0, b = a1, c = -a2, e = -a3;
+        const R num = a * a * b * b + a * b * (a + b) * (c + e) + c * e * (a * a + a * b + b * b);
+        return num / (((a + c) * (a + e)) * ((b + c) * (b + e)));
+    }
+    // n == 3: the near vertex a3 <= 0 sees the other three beyond
+    const R c = -a3, p = a0, q = a1, r = a2;
+    return R(1) - (c * c * c) / ((c + p) * (c + q) * (c + r));
+}
+
+// |vol6| of the part of tet (a,b,c,d) inside the slab 0 <= z <= 1.
+template <class R>
+ADECO_HD R zslab_vol6(const V3<R>& a, const V3<R>& b, const V3<R>& c, const V3<R>& d) {
+    const R w = absr(vol6(a, b, c, d));
+    const R below = beyond_fraction(-a.z, -b.z, -c.z, -d.z);
+    const R above = beyond_fraction(a.z - R(1), b.z - R(1), c.z - R(1), d.z - R(1));
+    const R f = R(1) - below - above;
+    return f > R(0) ? w * f : R(0);
+}
+
+// One clip stage: the part of tet t on the inside of plane (axis, value c, keep >= c when ge)
+// written as a point sequence seq[0..5] with `count` sub-tets seq[k..k+3], k < count.
+template <class R>
+struct Stage {
+    V3<R> s[6];
+    int count;
+};
+
+template <class R>
+ADECO_HD V3<R> cut_point(const V3<R>& in, R din, const V3<R>& out, R dout, int axis, R c) {
+    const R t = din / (din - dout);  // din > 0 >= dout: t in (0, 1]
+    V3<R> p = mk(in.x + t * (out.x - in.x), in.y + t * (out.y - in.y), in.z + t * (out.z - in.z));
+    set_comp(p, axis, c);
+    return p;
+}
+
+template <class R>
+ADECO_HD void clip_stage(const V3<R> t[4], int axis, R c, bool ge, Stage<R>& st) {
+    R d[4];
+    for (int i = 0; i < 4; ++i) {
+        const R x = comp(t[i], axis);
+        d[i] = ge ? (x - c) : (c - x);
+    }
+    const int m = int(d[0] > R(0)) | (int(d[1] > R(0)) << 1) | (int(d[2] > R(0)) << 2) | (int(d[3] > R(0)) << 3);
+    const int n = (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1) + ((m >> 3) & 1);
+    if (n == 0) { st.count = 0; return; }
+    if (n == 4) {
+        st.s[0] = t[0]; st.s[1] = t[1]; st.s[2] = t[2]; st.s[3] = t[3];
+        st.count = 1;
+        return;
+    }
+    // canonical order: inside vertices first
+    int ord[4];
+    int k = 0;
+    for (int i = 0; i < 4; ++i) if (m & (1 << i)) ord[k++] = i;
+    for (int i = 0; i < 4; ++i) if (!(m & (1 << i))) ord[k++] = i;
+    V3<R> p[4];
+    R dd[4];
+    for (int i = 0; i < 4; ++i) { p[i] = t[ord[i]]; dd[i] = d[ord[i]]; }
+    if (n == 1) {
+        st.s[0] = p[0];
+        st.s[1] = cut_point(p[0], dd[0], p[1], dd[1], axis, c);
+        st.s[2] = cut_point(p[0], dd[0], p[2], dd[2], axis, c);
+        st.s[3] = cut_point(p[0], dd[0], p[3], dd[3], axis, c);
+        st.count = 1;
+    } else if (n == 3) {  // prism (a,b,c) - (p_ao, p_bo, p_co)
+        st.s[0] = p[0]; st.s[1] = p[1]; st.s[2] = p[2];
+        st.s[3] = cut_point(p[0], dd[0], p[3], dd[3], axis, c);
+        st.s[4] = cut_point(p[1], dd[1], p[3], dd[3], axis, c);
+        st.s[5] = cut_point(p[2], dd[2], p[3], dd[3], axis, c);
+        st.count = 3;
+    } else {  // wedge (a, p_ac, p_ad) - (b, p_bc, p_bd)
+        st.s[0] = p[0];
+        st.s[1] = cut_point(p[0], dd[0], p[2], dd[2], axis, c);
+        st.s[2] = cut_point(p[0], dd[0], p[3], dd[3], axis, c);
+        st.s[3] = p[1];
+        st.s[4] = cut_point(p[1], dd[1], p[2], dd[2], axis, c);
+        st.s[5] = cut_point(p[1], dd[1], p[3], dd[3], axis, c);
+        st.count = 3;
+    }
+}
+
+template <class R>
+ADECO_HD void take(const Stage<R>& st, int k, V3<R> out[4]) {
+    for (int j = 0; j < 4; ++j) out[j] = sel(k == 0, st.s[j], sel(k == 1, st.s[j + 1], st.s[j + 2]));
+}
+
+// Which of the faces x>=0, x<=1, y>=0, y<=1 cut the tet (bit f set), or -1 if the tet lies
+// entirely outside one of them.
+template <class R>
+ADECO_HD int active_xy_faces(const V3<R> t[4]) {
+    int act = 0;
+    for (int f = 0; f < 4; ++f) {
+        const int axis = f >> 1;
+        const bool ge = (f & 1) == 0;
+        const R c = ge ? R(0) : R(1);
+        int nin = 0;
+        for (int i = 0; i < 4; ++i) {
+            const R x = comp(t[i], axis);
+            nin += int(ge ? (x - c) > R(0) : (c - x) > R(0));
+        }
+        if (nin == 0) return -1;
+        if (nin < 4) act |= 1 << f;
+    }
+    return act;
+}
+
+template <class R>
+ADECO_HD void face_of(int act, int level, int& axis, R& c, bool& ge) {
+    // level-th set bit of act
+    int f = 0, seen = 0;
+    for (int i = 0; i < 4; ++i)
+        if (act & (1 << i)) {
+            if (seen == level) f = i;
+            ++seen;
+        }
+    axis = f >> 1;
+    ge = (f & 1) == 0;
+    c = ge ? R(0) : R(1);
+}
+
+struct ClipCounters {
+    uint32_t stages, subtets;
+};
+
+// 6 x |t ∩ [0,1]^3| (t in cell units, any orientation).
+template <class R>
+ADECO_HD R clip_unit_cube_vol6(const V3<R> t[4], ClipCounters* cnt = nullptr) {
+    const int act = active_xy_faces(t);
+    if (act < 0) return R(0);
+    const int nact = (act & 1) + ((act >> 1) & 1) + ((act >> 2) & 1) + ((act >> 3) & 1);
+    if (nact == 0) {
+        if (cnt) cnt->subtets += 1;
+        return zslab_vol6(t[0], t[1], t[2], t[3]);
+    }
+    R vol = R(0);
+    int ax;
+    R c;
+    bool ge;
+    Stage<R> s1, s2, s3, s4;
+    V3<R> q1[4], q2[4], q3[4], q4[4];
+    face_of(act, 0, ax, c, ge);
+    clip_stage(t, ax, c, ge, s1);
+    if (cnt) cnt->stages += 1;
+    for (int k1 = 0; k1 < s1.count; ++k1) {
+        take(s1, k1, q1);
+        if (nact == 1) {
+            if (cnt) cnt->subtets += 1;
+            vol += zslab_vol6(q1[0], q1[1], q1[2], q1[3]);
+            continue;
+        }
+        face_of(act, 1, ax, c, ge);
+        clip_stage(q1, ax, c, ge, s2);
+        if (cnt) cnt->stages += 1;
+        for (int k2 = 0; k2 < s2.count; ++k2) {
+            take(s2, k2, q2);
+            if (nact == 2) {
+                if (cnt) cnt->subtets += 1;
+                vol += zslab_vol6(q2[0], q2[1], q2[2], q2[3]);
+                continue;
+            }
+            face_of(act, 2, ax, c, ge);
+            clip_stage(q2, ax, c, ge, s3);
+            if (cnt) cnt->stages += 1;
+            for (int k3 = 0; k3 < s3.count; ++k3) {
+                take(s3, k3, q3);
+                if (nact == 3) {
+                    if (cnt) cnt->subtets += 1;
+                    vol += zslab_vol6(q3[0], q3[1], q3[2], q3[3]);
+                    continue;
+                }
+                face_of(act, 3, ax, c, ge);
+                clip_stage(q3, ax, c, ge, s4);
+                if (cnt) cnt->stages += 1;
+                for (int k4 = 0; k4 < s4.count; ++k4) {
+                    take(s4, k4, q4);
+                    if (cnt) cnt->subtets += 1;
+                    vol += zslab_vol6(q4[0], q4[1], q4[2], q4[3]);
+                }
+            }
+        }
+    }
+    return vol;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Classification of a scaled cell-local tet against the unit cell, as the reference's
+// intersection_test (AdDepositTets2Grid.h:286-363): per tet face, the cube corners strictly on
+// the inner side; no corner inside one face -> no intersection; all corners inside every face ->
+// the cell lies in the tet; all tet vertices in [0,1]^3 -> the tet lies in the cell.
+enum CellClass : int { CLS_NONE = 0, CLS_TET_IN_CELL = 1, CLS_CELL_IN_TET = 2, CLS_PARTIAL = 3 };
+
+template <class R>
+ADECO_HD V3<R> cross(const V3<R>& a, const V3<R>& b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+template <class R>
+ADECO_HD V3<R> sub(const V3<R>& a, const V3<R>& b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+
+template <class R>
+ADECO_HD int classify_unit(const V3<R> s[4]) {
+    const V3<R> e01 = sub(s[1], s[0]), e02 = sub(s[2], s[0]), e03 = sub(s[3], s[0]);
+    const V3<R> e13 = sub(s[3], s[1]), e12 = sub(s[2], s[1]);
+    const V3<R> n[4] = {cross(e01, e02), cross(e02, e03), cross(e03, e01), cross(e13, e12)};
+    bool all_in = true;
+    for (int f = 0; f < 4; ++f) {
+        const V3<R> o = f == 3 ? s[1] : s[0];
+        int inside = 0;
+        for (int p = 0; p < 8; ++p) {
+            const R px = R((p >> 2) & 1), py = R((p >> 1) & 1), pz = R(p & 1);
+            const R v = (o.x - px) * n[f].x + (o.y - py) * n[f].y + (o.z - pz) * n[f].z;
+            inside += int(v > R(0));
+        }
+        if (inside == 0) return CLS_NONE;
+        all_in = all_in && inside == 8;
+    }
+    if (all_in) return CLS_CELL_IN_TET;
+    bool tin = true;
+    for (int i = 0; i < 4; ++i)
+        tin = tin && s[i].x >= R(0) && s[i].x <= R(1) && s[i].y >= R(0) && s[i].y <= R(1) &&
+              s[i].z >= R(0) && s[i].z <= R(1);
+    return tin ? CLS_TET_IN_CELL : CLS_PARTIAL;
+}
+
+}  // namespace adeco

@@ -1,0 +1,222 @@
+// Resident lattice and single-grid deposits (C ABI part 2).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/adeco.h"
+#include "deposit_tet.hpp"
+#include "kernels.hpp"
+#include "lattice_impl.hpp"
+#include "runtime.hpp"
+
+namespace adeco {
+
+namespace {
+
+// Per-block float min/max of the three coordinate arrays; out[6*block + 2a] = min, +1 = max.
+__global__ void __launch_bounds__(256) k_minmax(const float* __restrict__ x, const float* __restrict__ y,
+                                                const float* __restrict__ z, int64_t n, float* out) {
+    __shared__ float sm[6][256];
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const float p[3] = {x[i], y[i], z[i]};
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = p[a] < mn[a] ? p[a] : mn[a];
+            mx[a] = p[a] > mx[a] ? p[a] : mx[a];
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        sm[2 * a][threadIdx.x] = mn[a];
+        sm[2 * a + 1][threadIdx.x] = mx[a];
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s)
+            for (int a = 0; a < 3; ++a) {
+                const float o0 = sm[2 * a][threadIdx.x + s], o1 = sm[2 * a + 1][threadIdx.x + s];
+                if (o0 < sm[2 * a][threadIdx.x]) sm[2 * a][threadIdx.x] = o0;
+                if (o1 > sm[2 * a + 1][threadIdx.x]) sm[2 * a + 1][threadIdx.x] = o1;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[6 * blockIdx.x + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+}  // namespace
+
+adeco_status lattice_bbox_device(adeco_lattice_s* L, float out6[6]) {
+    DeviceState& ds = *L->ds;
+    const int nb = 512;
+    adeco_status st = ds.scratch.reserve(sizeof(float) * 6 * nb);
+    if (st) return st;
+    hipLaunchKernelGGL(k_minmax, dim3(nb), dim3(256), 0, ds.stream, L->d_x, L->d_y, L->d_z, L->n_vertices,
+                       ds.scratch.as<float>());
+    hipError_t e = hipGetLastError();
+    std::vector<float> part(6 * nb);
+    if (e == hipSuccess) e = hipMemcpyAsync(part.data(), ds.scratch.ptr, sizeof(float) * 6 * nb, hipMemcpyDeviceToHost, ds.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "lattice bbox");
+    for (int a = 0; a < 3; ++a) {
+        float mn = FLT_MAX, mx = -FLT_MAX;
+        for (int b = 0; b < nb; ++b) {
+            mn = std::min(mn, part[6 * b + 2 * a]);
+            mx = std::max(mx, part[6 * b + 2 * a + 1]);
+        }
+        out6[2 * a] = mn;
+        out6[2 * a + 1] = mx;
+    }
+    return ADECO_OK;
+}
+
+}  // namespace adeco
+
+using namespace adeco;
+
+extern "C" {
+
+adeco_status adeco_lattice_create(int device_id, const float* x, const float* y, const float* z, int npx,
+                                  int npy, int npz, adeco_lattice* out) {
+    if (!out || !x || !y || !z) return fail(ADECO_EINVAL, "lattice_create(): null pointer argument");
+    *out = nullptr;
+    if (npx < 2 || npy < 2 || npz < 2) return fail(ADECO_EINVAL, "lattice_create(): need at least 2 vertices per axis");
+    const int64_t n = int64_t(npx) * npy * npz;
+    if (n > (int64_t(1) << 31) / 6)  // tet ids are int32 in the binned work lists
+        return fail(ADECO_EOVERFLOW, "lattice_create(): lattice too large for 32-bit tet ids");
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]))
+            return fail(ADECO_EINVAL, "lattice_create(): non-finite vertex coordinate");
+    adeco_status st = check_device(device_id);
+    if (st) return st;
+    DeviceState& ds = device_state(device_id);
+    std::lock_guard<std::mutex> lk(ds.mutex);
+    if ((st = ds.ensure(device_id))) return st;
+    hipError_t e = hipSetDevice(device_id);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    auto* L = new adeco_lattice_s();
+    L->device = device_id;
+    L->ds = &ds;
+    L->np[0] = npx; L->np[1] = npy; L->np[2] = npz;
+    L->n_vertices = n;
+    const size_t bytes = size_t(n) * sizeof(float);
+    e = hipMalloc(&L->d_x, bytes);
+    if (e == hipSuccess) e = hipMalloc(&L->d_y, bytes);
+    if (e == hipSuccess) e = hipMalloc(&L->d_z, bytes);
+    if (e != hipSuccess) {
+        delete L;
+        return fail(ADECO_ENOMEM, std::string("lattice_create(): ") + hipGetErrorString(e));
+    }
+    e = hipMemcpyAsync(L->d_x, x, bytes, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(L->d_y, y, bytes, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(L->d_z, z, bytes, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
+    if (e != hipSuccess) {
+        delete L;
+        return hip_fail(e, "lattice upload");
+    }
+    *out = L;
+    return ADECO_OK;
+}
+
+adeco_status adeco_lattice_destroy(adeco_lattice L) {
+    if (!L) return ADECO_OK;
+    std::lock_guard<std::mutex> lk(L->ds->mutex);
+    (void)hipSetDevice(L->device);
+    delete L;
+    return ADECO_OK;
+}
+
+adeco_status adeco_lattice_bbox(adeco_lattice L, float* bbox6) {
+    if (!L || !bbox6) return fail(ADECO_EINVAL, "lattice_bbox(): null argument");
+    std::lock_guard<std::mutex> lk(L->ds->mutex);
+    hipError_t e = hipSetDevice(L->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    return lattice_bbox_device(L, bbox6);
+}
+
+adeco_status adeco_lattice_deposit_grid(adeco_lattice L, int decomposition, double tet_mass,
+                                        const int32_t* tet_ids, int64_t n_tets, const float* bbox6, int dims,
+                                        int mode, int precision, double* grid_out, uint64_t* stats) {
+    g_last_kernel_ms = 0.0;
+    if (!L || !bbox6 || !grid_out) return fail(ADECO_EINVAL, "lattice_deposit_grid(): null argument");
+    if (decomposition < 0 || decomposition > 2) return fail(ADECO_EINVAL, "lattice_deposit_grid(): unknown decomposition");
+    if (dims < 1 || !valid_bbox(bbox6)) return fail(ADECO_EINVAL, "lattice_deposit_grid(): invalid grid");
+    if (mode != ADECO_MODE_PLAIN && mode != ADECO_MODE_BATCHED) return fail(ADECO_EINVAL, "lattice_deposit_grid(): unknown mode");
+    if (precision < 0 || precision > 2) return fail(ADECO_EINVAL, "lattice_deposit_grid(): unknown precision");
+    const int64_t all = 6 * int64_t(L->np[0] - 1) * (L->np[1] - 1) * (L->np[2] - 1);
+    if (!tet_ids) n_tets = all;
+    if (n_tets < 0) return fail(ADECO_EINVAL, "lattice_deposit_grid(): negative tet count");
+    if (tet_ids)
+        for (int64_t i = 0; i < n_tets; ++i)
+            if (tet_ids[i] < 0 || tet_ids[i] >= all) return fail(ADECO_EINVAL, "lattice_deposit_grid(): tet id out of range");
+    DeviceState& ds = *L->ds;
+    std::lock_guard<std::mutex> lk(ds.mutex);
+    hipError_t e = hipSetDevice(L->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const size_t ncell = size_t(dims) * dims * dims;
+    adeco_status st = ds.grid.reserve(ncell * sizeof(double));
+    if (st) return st;
+    if (tet_ids && (st = ds.pos.reserve(size_t(n_tets) * sizeof(int32_t) + 4))) return st;
+    if ((st = ds.stats.reserve(kNumStats * sizeof(unsigned long long)))) return st;
+    const GridGeom g = make_grid_geom(bbox6, dims);
+    e = launch_fill_f64(ds.grid.as<double>(), int64_t(ncell), 0.0, ds.stream);
+    if (e == hipSuccess && tet_ids)
+        e = hipMemcpyAsync(ds.pos.ptr, tet_ids, size_t(n_tets) * sizeof(int32_t), hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess && stats) e = hipMemsetAsync(ds.stats.ptr, 0, kNumStats * sizeof(unsigned long long), ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "deposit setup");
+    LatticeView lv{L->d_x, L->d_y, L->d_z, {L->np[0], L->np[1], L->np[2]}};
+    if ((st = timed_begin(ds))) return st;
+    e = launch_deposit_lattice_plain(lv, decomposition, tet_ids ? ds.pos.as<int32_t>() : nullptr, n_tets, g, tet_mass,
+                                     ds.grid.as<double>(), stats ? ds.stats.as<unsigned long long>() : nullptr, ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "deposit launch");
+    if ((st = timed_end(ds))) return st;
+    e = hipMemcpyAsync(grid_out, ds.grid.ptr, ncell * sizeof(double), hipMemcpyDeviceToHost, ds.stream);
+    if (e == hipSuccess && stats)
+        e = hipMemcpyAsync(stats, ds.stats.ptr, kNumStats * sizeof(unsigned long long), hipMemcpyDeviceToHost, ds.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "deposit download");
+    (void)mode;
+    (void)precision;
+    return ADECO_OK;
+}
+
+adeco_status adeco_deposit_soup(int device_id, const float* tets, int64_t n_tets, const float* bbox6, int dims,
+                                double tet_mass, int mode, int precision, double* grid_out, uint64_t* stats) {
+    g_last_kernel_ms = 0.0;
+    if (!tets || !bbox6 || !grid_out || n_tets < 0) return fail(ADECO_EINVAL, "deposit_soup(): invalid argument");
+    if (dims < 1 || !valid_bbox(bbox6)) return fail(ADECO_EINVAL, "deposit_soup(): invalid grid");
+    adeco_status st = check_device(device_id);
+    if (st) return st;
+    DeviceState& ds = device_state(device_id);
+    std::lock_guard<std::mutex> lk(ds.mutex);
+    if ((st = ds.ensure(device_id))) return st;
+    hipError_t e = hipSetDevice(device_id);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const size_t ncell = size_t(dims) * dims * dims;
+    if ((st = ds.grid.reserve(ncell * sizeof(double)))) return st;
+    if ((st = ds.pos.reserve(size_t(n_tets) * 12 * sizeof(float) + 4))) return st;
+    if ((st = ds.stats.reserve(kNumStats * sizeof(unsigned long long)))) return st;
+    const GridGeom g = make_grid_geom(bbox6, dims);
+    e = launch_fill_f64(ds.grid.as<double>(), int64_t(ncell), 0.0, ds.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ds.pos.ptr, tets, size_t(n_tets) * 12 * sizeof(float), hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess && stats) e = hipMemsetAsync(ds.stats.ptr, 0, kNumStats * sizeof(unsigned long long), ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "soup setup");
+    if ((st = timed_begin(ds))) return st;
+    e = launch_deposit_soup_plain(ds.pos.as<float>(), n_tets, g, tet_mass, ds.grid.as<double>(),
+                                  stats ? ds.stats.as<unsigned long long>() : nullptr, ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "soup launch");
+    if ((st = timed_end(ds))) return st;
+    e = hipMemcpyAsync(grid_out, ds.grid.ptr, ncell * sizeof(double), hipMemcpyDeviceToHost, ds.stream);
+    if (e == hipSuccess && stats)
+        e = hipMemcpyAsync(stats, ds.stats.ptr, kNumStats * sizeof(unsigned long long), hipMemcpyDeviceToHost, ds.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "soup download");
+    (void)mode;
+    (void)precision;
+    return ADECO_OK;
+}
+
+}  // extern "C"

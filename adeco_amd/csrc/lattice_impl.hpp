@@ -1,0 +1,27 @@
+// The resident-lattice handle behind adeco_lattice.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "runtime.hpp"
+
+struct adeco_lattice_s {
+    int device = -1;
+    adeco::DeviceState* ds = nullptr;
+    int np[3] = {0, 0, 0};
+    int64_t n_vertices = 0;
+    float* d_x = nullptr;
+    float* d_y = nullptr;
+    float* d_z = nullptr;
+    ~adeco_lattice_s() {
+        if (d_x) (void)hipFree(d_x);
+        if (d_y) (void)hipFree(d_y);
+        if (d_z) (void)hipFree(d_z);
+    }
+};
+
+namespace adeco {
+adeco_status lattice_bbox_device(adeco_lattice_s* L, float out6[6]);
+}

@@ -1,0 +1,271 @@
+// Octree refinement counts and per-leaf tet binning on the GPU (C ABI part 3).
+//
+// The reference builds its octree depth-first on the host, summing per node a float estimate of
+// the tets of each meta-cell it overlaps (AdOctreeGeneration.h:76-258, MetaMesh
+// AdTypeDefinitions.h:715-1022).  Here the float work runs on the GPU, one level at a time:
+//   * k_meta_bbox: float AABB of each meta-cell's <= 8^3 vertices (min/max: exact in any order);
+//   * k_estimate_root / k_refine_level: for every (meta-cell, node) pair the strict-overlap test
+//     and est = uint64(float(6*cells) * vol(node ∩ bbox_m) / vol(bbox_m)) with the reference's
+//     float operation order (no fused multiply-add can form: the expression has no additions
+//     after products), summed per node with 64-bit integer atomics, which is exact in any order.
+// Which nodes refine does not depend on the traversal order, so the host only replays the
+// reference's DFS stack (children pushed 0..7, popped 7..0) to number the nodes and list leaves
+// exactly as the reference does.  Binning then gives every leaf the lattice tets whose AABB
+// strictly overlaps it (a superset of nothing: exactly the tets that can deposit mass there).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <list>
+#include <stack>
+#include <vector>
+
+#include "../../include/adeco.h"
+#include "deposit_tet.hpp"
+#include "kernels.hpp"
+#include "lattice_impl.hpp"
+#include "octree_impl.hpp"
+#include "runtime.hpp"
+
+namespace adeco {
+
+namespace {
+
+__device__ __forceinline__ bool fstrict(const float* a, const float* b) {  // boxes as x0,x1,y0,y1,z0,z1
+    return a[0] < b[1] && b[0] < a[1] && a[2] < b[3] && b[2] < a[3] && a[4] < b[5] && b[4] < a[5];
+}
+
+// AdTypeDefinitions.h:812-825 (AABBox::getVolume is ((x1-x0)*(y1-y0))*(z1-z0) in float)
+__device__ __forceinline__ unsigned long long estimate(const float* box, const float* mb, unsigned cells) {
+    float it[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (fstrict(box, mb)) {
+        for (int a = 0; a < 3; ++a) {
+            it[2 * a] = box[2 * a] < mb[2 * a] ? mb[2 * a] : box[2 * a];
+            it[2 * a + 1] = box[2 * a + 1] < mb[2 * a + 1] ? box[2 * a + 1] : mb[2 * a + 1];
+        }
+    }
+    const float vi = __fmul_rn(__fmul_rn(__fsub_rn(it[1], it[0]), __fsub_rn(it[3], it[2])), __fsub_rn(it[5], it[4]));
+    const float vm = __fmul_rn(__fmul_rn(__fsub_rn(mb[1], mb[0]), __fsub_rn(mb[3], mb[2])), __fsub_rn(mb[5], mb[4]));
+    const float e = __fdiv_rn(__fmul_rn(float(6ull * cells), vi), vm);
+    // float -> uint64 truncation; NaN (flat meta-cell) and negatives map to 0
+    return e >= 1.f ? (unsigned long long)(e) : 0ull;
+}
+
+// one wave (64 lanes) per meta-cell
+__global__ void __launch_bounds__(64) k_meta_bbox(const float* __restrict__ x, const float* __restrict__ y,
+                                                  const float* __restrict__ z, int npx, int npy, int npz,
+                                                  int mdx, int mdy, int mdz, float* __restrict__ out_box,
+                                                  unsigned* __restrict__ out_cells) {
+    const int m = blockIdx.x;
+    const int cx = m % mdx, cy = (m / mdx) % mdy, cz = m / (mdx * mdy);
+    const int lo[3] = {7 * cx, 7 * cy, 7 * cz};
+    const int d[3] = {cx + 1 == mdx ? npx - lo[0] : 8, cy + 1 == mdy ? npy - lo[1] : 8, cz + 1 == mdz ? npz - lo[2] : 8};
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    const int nv = d[0] * d[1] * d[2];
+    for (int v = threadIdx.x; v < nv; v += 64) {
+        const int i = v % d[0], j = (v / d[0]) % d[1], k = v / (d[0] * d[1]);
+        const int64_t g = (lo[0] + i) + int64_t(npx) * ((lo[1] + j) + int64_t(npy) * (lo[2] + k));
+        const float p[3] = {x[g], y[g], z[g]};
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = p[a] < mn[a] ? p[a] : mn[a];
+            mx[a] = p[a] > mx[a] ? p[a] : mx[a];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; ++a) {
+            const float o0 = __shfl_down(mn[a], off, 64), o1 = __shfl_down(mx[a], off, 64);
+            mn[a] = o0 < mn[a] ? o0 : mn[a];
+            mx[a] = o1 > mx[a] ? o1 : mx[a];
+        }
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 3; ++a) {
+            out_box[6 * m + 2 * a] = mn[a];
+            out_box[6 * m + 2 * a + 1] = mx[a];
+        }
+        out_cells[m] = unsigned(d[0] - 1) * unsigned(d[1] - 1) * unsigned(d[2] - 1);
+    }
+}
+
+__global__ void k_estimate_root(const float* __restrict__ mbox, const unsigned* __restrict__ mcells, int M,
+                                RootBox rb, unsigned long long* count) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long e = 0;
+    if (m < M) e = estimate(rb.b, mbox + 6 * m, mcells[m]);
+    // wave reduction, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+    if ((threadIdx.x & 63) == 0 && e) atomicAdd(count, e);
+}
+
+// pairs of a refined level: (refined-parent slot, meta-cell); for each of the 8 children test
+// strict overlap, set a membership bit and add the child's estimate.
+__global__ void k_refine_level(const float* __restrict__ mbox, const unsigned* __restrict__ mcells,
+                               const int32_t* __restrict__ pair_slot, const int32_t* __restrict__ pair_mc,
+                               int64_t n_pairs, const float* __restrict__ child_boxes,
+                               unsigned char* __restrict__ flags, unsigned long long* __restrict__ child_count) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n_pairs) return;
+    const int s = pair_slot[p];
+    const int m = pair_mc[p];
+    const float* mb = mbox + 6 * m;
+    unsigned char f = 0;
+    for (int c = 0; c < 8; ++c) {
+        const float* cb = child_boxes + 6 * (8 * s + c);
+        if (fstrict(mb, cb)) {  // distribute_tets_to_children: intersection(meta_cell_bbox, child_bbox)
+            f |= (unsigned char)(1u << c);
+            const unsigned long long e = estimate(cb, mb, mcells[m]);
+            if (e) atomicAdd(child_count + 8 * s + c, e);
+        }
+    }
+    flags[p] = f;
+}
+
+// ---- per-tet binning into leaves (tree descent with a bounded explicit stack) ----
+constexpr int kMaxStack = 8 * kMaxOctreeDepth + 8;
+
+template <bool kFill>
+__global__ void __launch_bounds__(256) k_bin_tets(LatticeView lat, CornerTable ct, int64_t n_tets,
+                                                  const float* __restrict__ node_box, const int32_t* __restrict__ first_child,
+                                                  const int32_t* __restrict__ leaf_index, int* __restrict__ leaf_count,
+                                                  const int64_t* __restrict__ leaf_offset, int32_t* __restrict__ pairs) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n_tets) return;
+    V3<double> v[4];
+    lattice_tet_vertices(lat, ct, t, v);
+    // the tet AABB in float is exact (vertices are float)
+    float tb[6];
+    tb[0] = tb[1] = float(v[0].x);
+    tb[2] = tb[3] = float(v[0].y);
+    tb[4] = tb[5] = float(v[0].z);
+    for (int q = 1; q < 4; ++q) {
+        const float p[3] = {float(v[q].x), float(v[q].y), float(v[q].z)};
+        for (int a = 0; a < 3; ++a) {
+            tb[2 * a] = p[a] < tb[2 * a] ? p[a] : tb[2 * a];
+            tb[2 * a + 1] = p[a] > tb[2 * a + 1] ? p[a] : tb[2 * a + 1];
+        }
+    }
+    if (!fstrict(tb, node_box)) return;
+    int stack[kMaxStack];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const int nd = stack[--sp];
+        const int fc = first_child[nd];
+        if (fc < 0) {
+            const int leaf = leaf_index[nd];
+            const int pos = atomicAdd(leaf_count + leaf, 1);
+            if (kFill) pairs[leaf_offset[leaf] + pos] = int32_t(t);
+            continue;
+        }
+        for (int c = 0; c < 8 && sp < kMaxStack; ++c)
+            if (fstrict(tb, node_box + 6 * (fc + c))) stack[sp++] = fc + c;
+    }
+}
+
+}  // namespace
+
+adeco_status octree_meta_bbox(adeco_lattice_s* L, const int md[3], float* d_box, unsigned* d_cells) {
+    DeviceState& ds = *L->ds;
+    const int M = md[0] * md[1] * md[2];
+    hipLaunchKernelGGL(k_meta_bbox, dim3(M), dim3(64), 0, ds.stream, L->d_x, L->d_y, L->d_z, L->np[0], L->np[1],
+                       L->np[2], md[0], md[1], md[2], d_box, d_cells);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ADECO_OK : hip_fail(e, "meta-cell bbox kernel");
+}
+
+adeco_status octree_root_count(DeviceState& ds, const float* d_box, const unsigned* d_cells, int M, const float box[6],
+                               unsigned long long* d_count, unsigned long long* h_count) {
+    RootBox rb;
+    std::memcpy(rb.b, box, sizeof(rb.b));
+    hipError_t e = hipMemsetAsync(d_count, 0, sizeof(unsigned long long), ds.stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_estimate_root, dim3((M + 255) / 256), dim3(256), 0, ds.stream, d_box, d_cells, M, rb, d_count);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h_count, d_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, ds.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
+    return e == hipSuccess ? ADECO_OK : hip_fail(e, "root estimate");
+}
+
+adeco_status octree_refine_level(DeviceState& ds, const float* d_box, const unsigned* d_cells,
+                                 const std::vector<int32_t>& slot, const std::vector<int32_t>& mc,
+                                 const std::vector<float>& child_boxes, std::vector<unsigned char>& flags,
+                                 std::vector<unsigned long long>& child_count) {
+    const int64_t n = int64_t(slot.size());
+    const int64_t nchild = int64_t(child_boxes.size() / 6);
+    flags.assign(size_t(n), 0);
+    child_count.assign(size_t(nchild), 0);
+    if (n == 0) return ADECO_OK;
+    DevBuf d_slot, d_mc, d_cb, d_flags, d_cnt;
+    adeco_status st;
+    if ((st = d_slot.reserve(n * 4)) || (st = d_mc.reserve(n * 4)) || (st = d_cb.reserve(child_boxes.size() * 4)) ||
+        (st = d_flags.reserve(n)) || (st = d_cnt.reserve(nchild * 8)))
+        return st;
+    hipError_t e = hipMemcpyAsync(d_slot.ptr, slot.data(), n * 4, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_mc.ptr, mc.data(), n * 4, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_cb.ptr, child_boxes.data(), child_boxes.size() * 4, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_cnt.ptr, 0, nchild * 8, ds.stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_refine_level, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ds.stream, d_box, d_cells,
+                           d_slot.as<int32_t>(), d_mc.as<int32_t>(), n, d_cb.as<float>(), d_flags.as<unsigned char>(),
+                           d_cnt.as<unsigned long long>());
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(flags.data(), d_flags.ptr, n, hipMemcpyDeviceToHost, ds.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(child_count.data(), d_cnt.ptr, nchild * 8, hipMemcpyDeviceToHost, ds.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
+    return e == hipSuccess ? ADECO_OK : hip_fail(e, "octree level refinement");
+}
+
+adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
+    adeco_lattice_s* L = T->L;
+    DeviceState& ds = *L->ds;
+    const int64_t n_nodes = int64_t(T->nodes.size());
+    const int64_t n_leaves = int64_t(T->leaves_dfs.size());
+    std::vector<float> boxes(6 * n_nodes);
+    std::vector<int32_t> first(n_nodes), leaf_index(n_nodes, -1);
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        T->node_box(int(i), &boxes[6 * i]);
+        first[i] = T->nodes[i].first_child;
+    }
+    for (int64_t l = 0; l < n_leaves; ++l) leaf_index[T->leaves_dfs[l]] = int32_t(l);
+    adeco_status st;
+    if ((st = T->d_node_box.reserve(boxes.size() * 4)) || (st = T->d_first.reserve(n_nodes * 4)) ||
+        (st = T->d_leaf_index.reserve(n_nodes * 4)) || (st = T->d_leaf_count.reserve(n_leaves * 4 + 4)) ||
+        (st = T->d_leaf_offset.reserve((n_leaves + 1) * 8)))
+        return st;
+    hipError_t e = hipMemcpyAsync(T->d_node_box.ptr, boxes.data(), boxes.size() * 4, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(T->d_first.ptr, first.data(), n_nodes * 4, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(T->d_leaf_index.ptr, leaf_index.data(), n_nodes * 4, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemsetAsync(T->d_leaf_count.ptr, 0, n_leaves * 4, ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "binning setup");
+    const int64_t n_tets = 6 * int64_t(L->np[0] - 1) * (L->np[1] - 1) * (L->np[2] - 1);
+    const LatticeView lv{L->d_x, L->d_y, L->d_z, {L->np[0], L->np[1], L->np[2]}};
+    const CornerTable ct = corner_table(decomposition);
+    const unsigned nb = unsigned((n_tets + 255) / 256);
+    hipLaunchKernelGGL(k_bin_tets<false>, dim3(nb), dim3(256), 0, ds.stream, lv, ct, n_tets, T->d_node_box.as<float>(),
+                       T->d_first.as<int32_t>(), T->d_leaf_index.as<int32_t>(), T->d_leaf_count.as<int>(),
+                       (const int64_t*)nullptr, (int32_t*)nullptr);
+    e = hipGetLastError();
+    std::vector<int> cnt(n_leaves);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), T->d_leaf_count.ptr, n_leaves * 4, hipMemcpyDeviceToHost, ds.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "binning count pass");
+    T->leaf_tet_offset.assign(n_leaves + 1, 0);
+    for (int64_t l = 0; l < n_leaves; ++l) T->leaf_tet_offset[l + 1] = T->leaf_tet_offset[l] + cnt[l];
+    const int64_t total = T->leaf_tet_offset[n_leaves];
+    if ((st = T->d_pairs.reserve(size_t(total) * 4 + 4))) return st;
+    e = hipMemcpyAsync(T->d_leaf_offset.ptr, T->leaf_tet_offset.data(), (n_leaves + 1) * 8, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemsetAsync(T->d_leaf_count.ptr, 0, n_leaves * 4, ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "binning offsets");
+    hipLaunchKernelGGL(k_bin_tets<true>, dim3(nb), dim3(256), 0, ds.stream, lv, ct, n_tets, T->d_node_box.as<float>(),
+                       T->d_first.as<int32_t>(), T->d_leaf_index.as<int32_t>(), T->d_leaf_count.as<int>(),
+                       T->d_leaf_offset.as<int64_t>(), T->d_pairs.as<int32_t>());
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "binning fill pass");
+    T->binned_decomposition = decomposition;
+    return ADECO_OK;
+}
+
+}  // namespace adeco

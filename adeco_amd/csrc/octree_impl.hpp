@@ -1,0 +1,66 @@
+// The octree handle behind adeco_octree and the host/device split of its construction.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "lattice_impl.hpp"
+#include "runtime.hpp"
+
+namespace adeco {
+
+constexpr int kMaxOctreeDepth = 20;
+
+struct RootBox {
+    float b[6];
+};
+
+struct ONode {
+    float ll[3];
+    float ext;
+    int level;
+    int parent;
+    int first_child;           // -1 when unrefined
+    unsigned long long count;  // num_tets_covering_node_estimate (== num_tets_on_proc_estimate[0] on one rank)
+};
+
+}  // namespace adeco
+
+struct adeco_octree_s {
+    adeco_lattice_s* L = nullptr;
+    float root_bbox[6];  // global bbox the root is built from
+    std::vector<adeco::ONode> nodes;  // reference (DFS) numbering
+    std::vector<int32_t> leaves_dfs, leaves_order;
+    std::vector<int64_t> leaf_mc_offset;
+    std::vector<int32_t> leaf_mc_ids;
+    int n_meta = 0;
+    // binning: per-leaf lists of lattice tet ids (device), offsets (host copy)
+    int binned_decomposition = -1;
+    std::vector<int64_t> leaf_tet_offset;
+    adeco::DevBuf d_mc_box, d_mc_cells, d_node_box, d_first, d_leaf_index, d_leaf_count, d_leaf_offset, d_pairs;
+    // leaf grids of the last deposit (device, FP64, leaf-major in DFS leaf order)
+    int grid_dims = 0;
+    adeco::DevBuf d_grids, d_leaf_geom, d_work;
+
+    // (ll, ll + ext) per axis in float, as AdOctreeGeneration::getBBox (AdOctreeGeneration.h:67-73)
+    void node_box(int id, float out[6]) const {
+        const adeco::ONode& n = nodes[id];
+        for (int a = 0; a < 3; ++a) {
+            out[2 * a] = n.ll[a];
+            out[2 * a + 1] = n.ll[a] + n.ext;
+        }
+    }
+};
+
+namespace adeco {
+
+adeco_status octree_meta_bbox(adeco_lattice_s* L, const int md[3], float* d_box, unsigned* d_cells);
+adeco_status octree_root_count(DeviceState& ds, const float* d_box, const unsigned* d_cells, int M, const float box[6],
+                               unsigned long long* d_count, unsigned long long* h_count);
+adeco_status octree_refine_level(DeviceState& ds, const float* d_box, const unsigned* d_cells,
+                                 const std::vector<int32_t>& slot, const std::vector<int32_t>& mc,
+                                 const std::vector<float>& child_boxes, std::vector<unsigned char>& flags,
+                                 std::vector<unsigned long long>& child_count);
+adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition);
+
+}  // namespace adeco

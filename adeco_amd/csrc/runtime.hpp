@@ -1,0 +1,50 @@
+// Host-runtime internals shared by the translation units of libadeco_amd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <mutex>
+#include <string>
+
+#include "../../include/adeco.h"
+
+namespace adeco {
+
+extern thread_local std::string g_last_error;
+extern thread_local double g_last_kernel_ms;
+
+adeco_status fail(adeco_status st, const std::string& msg);
+adeco_status hip_fail(hipError_t e, const char* what);
+adeco_status check_device(int dev);
+bool valid_bbox(const float* b);
+
+// A grow-only device allocation reused across calls.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    adeco_status reserve(size_t nbytes);
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf();
+};
+
+struct DeviceState {
+    int id = -1;
+    bool ready = false;
+    std::mutex mutex;  // serialises calls that share this device's buffers and stream
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t total_mem = 0;
+    DevBuf pos, grid, stats, scratch;
+    adeco_status ensure(int dev);
+};
+
+DeviceState& device_state(int dev);
+size_t memory_budget(const DeviceState& ds);
+adeco_status timed_begin(DeviceState& ds);
+adeco_status timed_end(DeviceState& ds);
+
+}  // namespace adeco

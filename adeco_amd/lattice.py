@@ -1,0 +1,190 @@
+"""Seeded synthetic particle lattices for the mass-deposit path.
+
+A lattice of N^3 Lagrangian cubes is stored as an (N+1)^3 vertex lattice in ColumnMajorOrder
+(index i + nx*(j + ny*k), i fastest; the reference's id order, AdTypeDefinitions.h:184-220).  The
+last vertex layer is the periodic image of the first, unwrapped by +L, so the lattice holds exactly
+6*N^3 tets and the reference's "cells between existing vertices" rule (AdTypeDefinitions.h:656)
+sees N^3 cubes with periodic_boundaries=false.
+
+Displacements are a curl-free Gaussian random field s = -grad(phi) whose per-component spectrum is
+P_s(k) ~ k^-2 exp(-k^2 sigma^2), sigma = 2 lattice spacings, scaled so the 3-D rms |s| equals the
+requested number of lattice spacings.  Optional spherical infall centres pull points towards
+seeded centres (halo-like caustics) and seeded near-degenerate tets can be generated as a tet soup.
+Positions are float32.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+BOX = 100.0  # Mpc/h
+
+
+@dataclasses.dataclass
+class Lattice:
+    """An unwrapped (n+1)^3 vertex lattice (SoA float32, i fastest)."""
+
+    n: int  # cubes per axis
+    box: float
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+
+    @property
+    def dims(self):
+        return (self.n + 1, self.n + 1, self.n + 1)
+
+    @property
+    def num_tets(self) -> int:
+        return 6 * self.n ** 3
+
+    def bbox(self):
+        """Float min/max over all vertices: x0,x1,y0,y1,z0,z1 (AdPointDataReaders.h:90-151)."""
+        return np.array([self.x.min(), self.x.max(), self.y.min(), self.y.max(),
+                         self.z.min(), self.z.max()], dtype=np.float32)
+
+
+def _displacement_field(n: int, box: float, rms_spacings: float, sigma_spacings: float,
+                        rng: np.random.Generator) -> np.ndarray:
+    """Returns s[3, n, n, n] (z, y, x index order) in box units."""
+    delta = box / n
+    white = rng.standard_normal((n, n, n))
+    wk = np.fft.rfftn(white)
+    kz = 2.0 * np.pi * np.fft.fftfreq(n, d=delta)
+    ky = 2.0 * np.pi * np.fft.fftfreq(n, d=delta)
+    kx = 2.0 * np.pi * np.fft.rfftfreq(n, d=delta)
+    KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+    k2 = KX * KX + KY * KY + KZ * KZ
+    sigma = sigma_spacings * delta
+    with np.errstate(divide="ignore", invalid="ignore"):
+        amp = np.where(k2 > 0, np.exp(-0.5 * k2 * sigma * sigma) / k2, 0.0)
+    phik = wk * amp
+    s = np.empty((3, n, n, n))
+    for c, kc in enumerate((KX, KY, KZ)):
+        s[c] = np.fft.irfftn(-1j * kc * phik, s=(n, n, n), axes=(0, 1, 2))
+    rms = np.sqrt(np.mean(np.sum(s * s, axis=0)))
+    if rms > 0:
+        s *= rms_spacings * delta / rms
+    return s
+
+
+def make_lattice(n: int, rms_spacings: float = 1.0, seed: int = 1, box: float = BOX,
+                 sigma_spacings: float = 2.0, infall_centres: int = 0) -> Lattice:
+    """Seeded lattice of n^3 cubes; see the module docstring for the field."""
+    if n < 1:
+        raise ValueError("n must be >= 1")
+    rng = np.random.default_rng(seed)
+    delta = box / n
+    if rms_spacings > 0 and n >= 2:
+        s = _displacement_field(n, box, rms_spacings, sigma_spacings, rng)
+    else:
+        s = np.zeros((3, n, n, n))
+    idx = np.arange(n + 1)
+    wrap = idx % n
+    # vertex (i,j,k) -> q + s[k%n, j%n, i%n]; arrays laid out [k, j, i] so ravel() is i-fastest
+    sz = s[:, wrap][:, :, wrap][:, :, :, wrap]
+    K, J, I = np.meshgrid(idx, idx, idx, indexing="ij")
+    px = I * delta + sz[0]
+    py = J * delta + sz[1]
+    pz = K * delta + sz[2]
+    if infall_centres > 0:
+        px, py, pz = _apply_infall(px, py, pz, box, infall_centres, rng)
+    return Lattice(n=n, box=box, x=px.astype(np.float32).ravel(), y=py.astype(np.float32).ravel(),
+                   z=pz.astype(np.float32).ravel())
+
+
+def _apply_infall(px, py, pz, box, count, rng):
+    """Spherical infall: x -> x - A (x - c) w(|x - c| / R) with w(r) = (1 - r^2)^2 for r < 1.
+
+    Centres are uniform in the box, R ~ U[2, 8] Mpc/h and A ~ U[0.8, 1.6] (A > 1 inverts the
+    tets near the centre).  Distances use the minimum image so periodic images move together.
+    """
+    centres = rng.uniform(0.0, box, size=(count, 3))
+    radii = rng.uniform(2.0, 8.0, size=count)
+    amps = rng.uniform(0.8, 1.6, size=count)
+    pos = [px.copy(), py.copy(), pz.copy()]
+    for c, r, a in zip(centres, radii, amps):
+        d = [pos[i] - c[i] for i in range(3)]
+        d = [di - box * np.round(di / box) for di in d]
+        rr = np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2) / r
+        w = np.where(rr < 1.0, (1.0 - rr * rr) ** 2, 0.0)
+        for i in range(3):
+            pos[i] = pos[i] - a * d[i] * w
+    return pos[0], pos[1], pos[2]
+
+
+def make_degenerate_tets(count: int, seed: int = 4, box: float = BOX, cell: float = 100.0 / 128,
+                         ratio_range=(1e3, 1e6)) -> np.ndarray:
+    """Seeded near-degenerate tets as a float32 soup [count, 4, 3].
+
+    Each tet has a longest edge of about one `cell`, an edge-length ratio log-uniform in
+    `ratio_range` (long/short), a random orientation and a random centre in the box, giving
+    volumes down to ~1e-12 cell^3.
+    """
+    rng = np.random.default_rng(seed)
+    ratio = np.exp(rng.uniform(np.log(ratio_range[0]), np.log(ratio_range[1]), size=count))
+    long_len = cell * rng.uniform(0.3, 1.5, size=count)
+    mid_len = long_len / np.sqrt(ratio) * rng.uniform(0.5, 2.0, size=count)
+    short_len = long_len / ratio
+    # random rotations from normalised quaternions
+    q = rng.standard_normal((count, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    rot = np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1),
+    ], 1)
+    local = np.zeros((count, 4, 3))
+    local[:, 1, 0] = long_len
+    local[:, 2, 0] = long_len * rng.uniform(0.2, 0.8, size=count)
+    local[:, 2, 1] = mid_len
+    local[:, 3, 0] = long_len * rng.uniform(0.2, 0.8, size=count)
+    local[:, 3, 1] = mid_len * rng.uniform(0.2, 0.8, size=count)
+    local[:, 3, 2] = short_len
+    local -= local.mean(axis=1, keepdims=True)
+    world = np.einsum("nij,nvj->nvi", rot, local)
+    centre = rng.uniform(0.0, box, size=(count, 1, 3))
+    return (world + centre).astype(np.float32)
+
+
+META = 7  # cells per meta-cell edge (AdTypeDefinitions.h:743-747)
+
+
+def meta_cell_dims(np_xyz) -> tuple:
+    """Meta-cells per axis: ceil((N_vertices - 1) / 7) (AdTypeDefinitions.h:730-735)."""
+    return tuple(-(-(d - 1) // META) for d in np_xyz)
+
+
+def meta_patches(lat: Lattice, ids=None) -> np.ndarray:
+    """Meta-patch AoS buffers as the reference's MetaMesh::get_tets packs them.
+
+    Each patch is 8^3 vertices x 3 floats, vertex (i',j',k') at float offset 3*(i'+8*(j'+8*k')),
+    pre-filled with the 666.f sentinel for vertices beyond the lattice
+    (AdTypeDefinitions.h:762-810).  `ids` selects meta-cells (ColumnMajorOrder over the meta-cell
+    dims); default all, in id order.  Returns float32 [len(ids) * 1536].
+    """
+    npd = lat.dims
+    md = meta_cell_dims(npd)
+    total = md[0] * md[1] * md[2]
+    if ids is None:
+        ids = np.arange(total)
+    ids = np.asarray(ids, dtype=np.int64)
+    out = np.full((len(ids), 8, 8, 8, 3), 666.0, dtype=np.float32)  # [p, k', j', i', xyz]
+    X = lat.x.reshape(npd[2], npd[1], npd[0])
+    Y = lat.y.reshape(npd[2], npd[1], npd[0])
+    Z = lat.z.reshape(npd[2], npd[1], npd[0])
+    for p, m in enumerate(ids):
+        cx = m % md[0]
+        cy = (m // md[0]) % md[1]
+        cz = m // (md[0] * md[1])
+        lo = (META * cx, META * cy, META * cz)
+        hi = tuple(min(lo[a] + META + 1, npd[a]) for a in range(3))
+        sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+        e = (hi[2] - lo[2], hi[1] - lo[1], hi[0] - lo[0])
+        out[p, :e[0], :e[1], :e[2], 0] = X[sl]
+        out[p, :e[0], :e[1], :e[2], 1] = Y[sl]
+        out[p, :e[0], :e[1], :e[2], 2] = Z[sl]
+    return out.reshape(-1)

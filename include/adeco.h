@@ -1,0 +1,154 @@
+/*
+ * adeco.h — C ABI of the MI355X-native ADECO mass deposit (libadeco_amd.so).
+ *
+ * Plain C: pointers, sizes and status codes only.  Every call is synchronous unless stated, keeps
+ * no pointer to caller memory after it returns, and reports failures through an adeco_status plus
+ * a thread-local message (adeco_last_error); nothing calls exit().  All GPU work runs in
+ * hand-written HIP kernels for gfx950; there is no CPU fallback: without a usable GPU the
+ * compute entry points return ADECO_EDEVICE.
+ *
+ * Reference interfaces replaced (kaehlerr/ADECO, src/):
+ *   adeco_deposit_patches    DepositWrapper::deposit_tets            AdDeposit2Grid.h:82-91
+ *                            (CPU AdDeposit2Grid.cpp:67-125, CUDA AdDeposit2Grid.cu:229-584)
+ *   adeco_get_grid_size      DepositWrapper::get_cuda_grid_size      AdDeposit2Grid.h:72-75
+ *   adeco_get_block_size     DepositWrapper::get_cuda_block_size     AdDeposit2Grid.h:77-80
+ *   adeco_octree_*           MetaMesh + AdOctreeGeneration::generate_tree + Octree
+ *                            AdTypeDefinitions.h:715-1022, AdOctreeGeneration.h:76-258, AdOctree.h:63-392
+ *   adeco_deposit_leaves     deposit_leaf_nodes (one rank)           AdDynamicLoadBalancing.h:983-1323
+ *                            + gather_local_tets                     AdDynamicLoadBalancing.h:894-950
+ *   adeco_serve_*            (no counterpart: region-of-interest density service on the resident lattice)
+ */
+#ifndef ADECO_H
+#define ADECO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int adeco_status;
+#define ADECO_OK 0
+#define ADECO_EINVAL 1    /* invalid argument (the reference throws AdRuntimeException or asserts) */
+#define ADECO_EOVERFLOW 2 /* a size does not fit (grid larger than the memory budget, ...) */
+#define ADECO_ENOMEM 3    /* device or host allocation failed */
+#define ADECO_EDEVICE 4   /* no usable GPU, or a HIP runtime error */
+#define ADECO_EINTERNAL 5 /* internal consistency check failed */
+
+/* Tet decompositions of a Lagrangian lattice cube (6 tets each). */
+#define ADECO_DECOMP_REF_EFFECTIVE 0 /* what the reference deposit applies (AdDepositTets2Grid.h:947-972 x :213-218) */
+#define ADECO_DECOMP_REF_INTENDED 1  /* the same table in vert[] order (AdDepositTets2Grid.h:188-192): tiles the cube */
+#define ADECO_DECOMP_KUHN 2          /* six tets around the (0,0,0)-(1,1,1) diagonal */
+
+/* Deposit execution forms. */
+#define ADECO_MODE_PLAIN 0   /* one tet per thread, one launch per output grid */
+#define ADECO_MODE_BATCHED 1 /* persistent work-queue launch over all grids, load-balanced (tet, cell) work */
+
+/* Precision of the clip (accumulation is always FP64 on the device). */
+#define ADECO_PREC_FP64 0  /* all geometry in FP64 */
+#define ADECO_PREC_MIXED 1 /* FP32 clip with FP64 recompute where the FP32 error bound is not met */
+#define ADECO_PREC_FP32 2  /* FP32 clip only (for the precision study) */
+
+const char* adeco_last_error(void);
+int adeco_version(void);
+/* Number of visible GPUs, or -1 when the HIP runtime reports an error. */
+int adeco_device_count(void);
+
+/* ---- drop-in boundary: DepositWrapper::deposit_tets (AdDeposit2Grid.h:82-91) ----
+ * positions: concatenated meta-patches of 3*(m+1)^3 floats (AoS xyz, vertex (i,j,k) at float
+ * offset 3*(i+(m+1)*(j+(m+1)*k)), 666.f sentinel marks absent vertices); patches
+ * [patch_offset, patch_offset+num_patches) are deposited with tet mass 1 into the dims^3 grid over
+ * bbox = x0,x1,y0,y1,z0,z1, ADDED into grid_accum (float, index i+dims*(j+dims*k)).  The device
+ * accumulates in FP64 and the sum is rounded to float once when added.  use_specific_gpu selects
+ * device_id; otherwise the patches are split over all visible GPUs.  Positions beyond the memory
+ * budget are streamed in several passes. */
+adeco_status adeco_deposit_patches(int use_specific_gpu, size_t device_id, size_t patch_offset,
+                                   size_t num_patches, const float* positions, size_t positions_size,
+                                   int meta_cells_per_patch, const float* bbox, int dims,
+                                   float* grid_accum);
+int adeco_get_grid_size(void);
+int adeco_get_block_size(void);
+/* Device-memory budget per call in bytes (0 = 75% of device memory, as the reference's
+ * MAX_GPU_USAGE, AdDeposit2Grid.cu:271).  Lowering it forces multi-pass streaming. */
+adeco_status adeco_set_memory_budget(size_t bytes);
+
+/* ---- resident lattice ---- */
+typedef struct adeco_lattice_s* adeco_lattice;
+/* Uploads an (npx,npy,npz) SoA float vertex lattice (i fastest) to device_id once. */
+adeco_status adeco_lattice_create(int device_id, const float* x, const float* y, const float* z,
+                                  int npx, int npy, int npz, adeco_lattice* out);
+adeco_status adeco_lattice_destroy(adeco_lattice lat);
+/* Float min/max over all vertices (AdPointDataReaders.h:90-151), computed on the device. */
+adeco_status adeco_lattice_bbox(adeco_lattice lat, float* bbox6);
+
+/* Deposits lattice tets (all 6*(npx-1)(npy-1)(npz-1), or the listed tet ids) with tet_mass onto
+ * one dims^3 grid over bbox6; grid_out (host, double, dims^3) is overwritten.  stats (nullable,
+ * 11 x uint64) receives the work counters of a counting launch. */
+adeco_status adeco_lattice_deposit_grid(adeco_lattice lat, int decomposition, double tet_mass,
+                                        const int32_t* tet_ids, int64_t n_tets, const float* bbox6,
+                                        int dims, int mode, int precision, double* grid_out,
+                                        uint64_t* stats);
+
+/* Deposits a tet soup (float [n][4][3]) onto one grid (precision study inputs). */
+adeco_status adeco_deposit_soup(int device_id, const float* tets, int64_t n_tets, const float* bbox6,
+                                int dims, double tet_mass, int mode, int precision, double* grid_out,
+                                uint64_t* stats);
+
+/* Kernel time (ms, HIP events on the launch stream) of the calling thread's last deposit. */
+double adeco_last_kernel_ms(void);
+
+/* ---- octree on the resident lattice: MetaMesh + generate_tree + Octree, one rank ----
+ * Root = (bbox.min, max extent) of the lattice's float bbox; meta-cells of 7^3 cubes; a node is
+ * refined iff it strictly meets roi6 (null: the whole domain), its estimate count > 0 and
+ * > min(num_tets, max_tets_per_node), and level < max_level.  Node ids, the DFS leaf list, the
+ * leaf processing order and every count are those of the reference. */
+typedef struct adeco_octree_s* adeco_octree;
+adeco_status adeco_octree_build(adeco_lattice lat, const float* roi6, uint64_t max_tets_per_node, int max_level,
+                                adeco_octree* out);
+adeco_status adeco_octree_destroy(adeco_octree tree);
+int64_t adeco_octree_num_nodes(adeco_octree tree);
+int64_t adeco_octree_num_leaves(adeco_octree tree);
+int64_t adeco_octree_num_meta_cells(adeco_octree tree);
+adeco_status adeco_octree_bbox(adeco_octree tree, float* bbox6);
+/* Per node (any pointer may be null): lower-left corner (3 floats), extension, level, parent id
+ * (-1 for the root), first child id (-1 when unrefined) and num_tets_covering_node_estimate. */
+adeco_status adeco_octree_nodes(adeco_octree tree, float* ll, float* ext, int32_t* level, int32_t* parent,
+                                int32_t* first_child, uint64_t* count);
+/* Leaves in DFS order, in processing order, and their meta-cell id lists (CSR over DFS order,
+ * mc_offsets has num_leaves + 1 entries).  Returns the total list length (-1 on a null handle). */
+int64_t adeco_octree_leaves(adeco_octree tree, int32_t* dfs, int32_t* order, int64_t* mc_offsets, int32_t* mc_ids);
+/* Meta-cell float AABBs, 6 floats (x0,x1,y0,y1,z0,z1) per meta-cell. */
+adeco_status adeco_octree_meta_bboxes(adeco_octree tree, float* out);
+/* Bins every lattice tet (in the given decomposition) to the leaves its AABB strictly overlaps;
+ * leaf_tet_counts (nullable, num_leaves) receives the binned tets per leaf (DFS order). */
+adeco_status adeco_octree_bin(adeco_octree tree, int decomposition, int64_t* leaf_tet_counts);
+
+/* ---- leaf deposit: deposit_leaf_nodes for one rank ----
+ * Deposits every binned tet into its leaves' dims^3 FP64 grids (HBM-resident, DFS leaf order);
+ * mode and precision select the kernel form.  The kernel time is reported by
+ * adeco_last_kernel_ms. */
+adeco_status adeco_deposit_leaves(adeco_octree tree, int dims, int decomposition, double tet_mass, int mode,
+                                  int precision);
+/* Copies leaf grids [first_leaf, first_leaf + n_leaves) of the last deposit (FP64 mass per cell,
+ * i fastest, leaf-major in DFS leaf order). */
+adeco_status adeco_leaf_grids(adeco_octree tree, int64_t first_leaf, int64_t n_leaves, double* out);
+
+/* ---- self-test hooks: the shared host/device geometry evaluated on the host ----
+ * These exist so the CPU test suite can check the clip and classification code that the kernels
+ * compile; no deposit entry point ever calls them. */
+double adeco_selftest_clip_vol6_f64(const double* tet12);
+float adeco_selftest_clip_vol6_f32(const float* tet12);
+int adeco_selftest_classify_f64(const double* tet12);
+double adeco_selftest_beyond_fraction_f64(const double* d4);
+void adeco_selftest_decomposition(int decomposition, int32_t* corners24);
+/* The kernels' per-tet FP64 deposit routine run on the host over a tet soup (float [n][4][3]),
+ * accumulating into grid (double, dims^3). */
+adeco_status adeco_selftest_deposit_soup_f64(const float* tets, int64_t n_tets, const float* bbox6, int dims,
+                                             double tet_mass, double* grid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADECO_H */

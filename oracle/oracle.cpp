@@ -1,0 +1,269 @@
+// Test infrastructure only: the oracle's C ABI (oracle/liboracle.so), loaded by tests/, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker.  The product library
+// (adeco_amd/libadeco_amd.so) never links or loads it.
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "oracle_core.hpp"
+#include "oracle_octree.hpp"
+
+namespace oracle {
+thread_local OpCounts* g_ops = nullptr;
+}
+
+using namespace oracle;
+
+namespace {
+
+// Statistics block shared with Python (uint64 x 16):
+//  0 tets, 1 tets_active, 2 candidates, 3 aabb_pass, 4 no, 5 tet_inside, 6 cell_inside,
+//  7 partial, 8 nonzero, 9 faces, 10 edge_points, 11 subtet_vols, 12 flops (add+mul+div),
+//  13 add, 14 mul, 15 div
+void stats_out(const DepositStats& s, const OpCounts* ops, uint64_t* out) {
+    if (!out) return;
+    out[0] += s.tets; out[1] += s.tets_active; out[2] += s.candidates; out[3] += s.aabb_pass;
+    out[4] += s.no; out[5] += s.tet_inside; out[6] += s.cell_inside; out[7] += s.partial;
+    out[8] += s.nonzero; out[9] += s.clip.faces; out[10] += s.clip.edge_points;
+    out[11] += s.clip.subtet_vols;
+    if (ops) { out[12] += ops->flops(); out[13] += ops->add; out[14] += ops->mul; out[15] += ops->div; }
+}
+
+bool patches_ok(size_t patch_offset, size_t num_patches, size_t positions_size, int m) {
+    if (m < 1) return false;
+    const size_t fpp = 3 * size_t(m + 1) * (m + 1) * (m + 1);
+    if (positions_size % fpp) return false;
+    return patch_offset + num_patches <= positions_size / fpp;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Float-faithful restatement of DepositWrapper::deposit_tets (AdDeposit2Grid.cpp:67-125):
+// accumulates into a float grid in the reference's order.  Returns 0, or -1 on inconsistent
+// patch ranges (where the reference would read out of bounds).
+int oracle_deposit_patches_f32(size_t patch_offset, size_t num_patches, const float* positions,
+                               size_t positions_size, int m, const float* bbox, int dims, float* grid) {
+    if (positions_size == 0) return 0;
+    if (!patches_ok(patch_offset, num_patches, positions_size, m)) return -1;
+    const size_t fpp = 3 * size_t(m + 1) * (m + 1) * (m + 1);
+    const int cpp = m * m * m;
+    const Box<float> gb = bbox6<float>(bbox);
+    for (size_t p = 0; p < num_patches; ++p)
+        for (int c = 0; c < cpp; ++c)
+            deposit_patch_cell<float, float>(c, positions + (patch_offset + p) * fpp, m, gb, dims,
+                                             1.f, grid, nullptr);
+    return 0;
+}
+
+// The same path in FP64 (coordinates converted exactly from float), into a double grid.
+int oracle_deposit_patches_f64(size_t patch_offset, size_t num_patches, const float* positions,
+                               size_t positions_size, int m, const float* bbox, int dims,
+                               double tet_mass, double* grid, uint64_t* stats) {
+    if (positions_size == 0) return 0;
+    if (!patches_ok(patch_offset, num_patches, positions_size, m)) return -1;
+    const size_t fpp = 3 * size_t(m + 1) * (m + 1) * (m + 1);
+    const int cpp = m * m * m;
+    const Box<double> gb = bbox6<double>(bbox);
+    DepositStats st;
+    for (size_t p = 0; p < num_patches; ++p)
+        for (int c = 0; c < cpp; ++c)
+            deposit_patch_cell<double, double>(c, positions + (patch_offset + p) * fpp, m, gb, dims,
+                                               tet_mass, grid, stats ? &st : nullptr);
+    stats_out(st, nullptr, stats);
+    return 0;
+}
+
+// Deposit of lattice tets (or of the listed tet ids) onto one uniform grid, split over nthreads
+// worker threads with private grids summed at the end (the reference's worker grids,
+// AdDynamicLoadBalancing.h:489-524).  precision: 0 = float-faithful into a float grid (gridf),
+// 1 = FP64 into a double grid (gridd), 2 = FP64 with operation counting (stats[12..15]).
+int oracle_deposit_lattice(const float* x, const float* y, const float* z, const int* np, int decomp,
+                           const int32_t* tet_ids, int64_t n_tets, const float* bbox, int dims,
+                           double tet_mass, int precision, float* gridf, double* gridd, int nthreads,
+                           uint64_t* stats) {
+    int corners[6][4];
+    decomposition_corners(decomp, corners);
+    const int gd[3] = {dims, dims, dims};
+    const size_t ncell = size_t(dims) * dims * dims;
+    if (nthreads < 1) nthreads = 1;
+    std::vector<std::vector<float>> gf(nthreads);
+    std::vector<std::vector<double>> gdv(nthreads);
+    std::vector<std::vector<Counted<double>>> gc(nthreads);
+    std::vector<DepositStats> sts(nthreads);
+    std::vector<OpCounts> ops(nthreads);
+    std::atomic<int64_t> next(0);
+    const int64_t chunk = 4096;
+    auto work = [&](int tid) {
+        if (precision == 0) gf[tid].assign(ncell, 0.f);
+        else if (precision == 1) gdv[tid].assign(ncell, 0.);
+        else gc[tid].assign(ncell, Counted<double>(0.));
+        if (precision == 2) g_ops = &ops[tid];
+        for (;;) {
+            const int64_t b = next.fetch_add(chunk);
+            if (b >= n_tets) break;
+            const int64_t e = std::min(n_tets, b + chunk);
+            for (int64_t q = b; q < e; ++q) {
+                const int64_t t = tet_ids ? int64_t(tet_ids[q]) : q;
+                if (precision == 0) {
+                    float v[4][3];
+                    lattice_tet(x, y, z, np, corners, t, v);
+                    deposit_tet<float, float>(v, float(tet_mass), gd, bbox6<float>(bbox), gf[tid].data(),
+                                              stats ? &sts[tid] : nullptr);
+                } else if (precision == 1) {
+                    double v[4][3];
+                    lattice_tet(x, y, z, np, corners, t, v);
+                    deposit_tet<double, double>(v, tet_mass, gd, bbox6<double>(bbox), gdv[tid].data(),
+                                                stats ? &sts[tid] : nullptr);
+                } else {
+                    using C = Counted<double>;
+                    C v[4][3];
+                    double vd[4][3];
+                    lattice_tet(x, y, z, np, corners, t, vd);
+                    for (int i = 0; i < 4; ++i)
+                        for (int a = 0; a < 3; ++a) v[i][a] = C(vd[i][a]);
+                    Box<C> gb;
+                    for (int a = 0; a < 3; ++a) { gb.mn[a] = C(double(bbox[2 * a])); gb.mx[a] = C(double(bbox[2 * a + 1])); }
+                    deposit_tet<C, C>(v, C(tet_mass), gd, gb, gc[tid].data(), &sts[tid]);
+                }
+            }
+        }
+        g_ops = nullptr;
+    };
+    if (nthreads == 1) work(0);
+    else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nthreads; ++t) th.emplace_back(work, t);
+        for (auto& t : th) t.join();
+    }
+    for (int t = 0; t < nthreads; ++t) {
+        if (precision == 0) { for (size_t c = 0; c < ncell; ++c) gridf[c] += gf[t][c]; }
+        else if (precision == 1) { for (size_t c = 0; c < ncell; ++c) gridd[c] += gdv[t][c]; }
+        else { for (size_t c = 0; c < ncell; ++c) gridd[c] += gc[t][c].v; }
+        stats_out(sts[t], precision == 2 ? &ops[t] : nullptr, stats);
+    }
+    return 0;
+}
+
+// Deposit of a tet soup (float [n][4][3]) in FP64 onto one grid.
+int oracle_deposit_soup_f64(const float* tets, int64_t n_tets, const float* bbox, int dims,
+                            double tet_mass, double* grid, uint64_t* stats) {
+    const int gd[3] = {dims, dims, dims};
+    const Box<double> gb = bbox6<double>(bbox);
+    DepositStats st;
+    for (int64_t t = 0; t < n_tets; ++t) {
+        double v[4][3];
+        for (int i = 0; i < 4; ++i)
+            for (int a = 0; a < 3; ++a) v[i][a] = double(tets[12 * t + 3 * i + a]);
+        deposit_tet<double, double>(v, tet_mass, gd, gb, grid, stats ? &st : nullptr);
+    }
+    stats_out(st, nullptr, stats);
+    return 0;
+}
+
+// Unit-cube clip of one tet (12 coordinates) in float-faithful or FP64 arithmetic.
+float oracle_clip_unit_cube_f32(const float* t) {
+    float v[4][3];
+    std::memcpy(v, t, sizeof(v));
+    return clip_unit_cube<float>(v);
+}
+double oracle_clip_unit_cube_f64(const double* t, uint64_t* faces_edges_vols) {
+    double v[4][3];
+    std::memcpy(v, t, sizeof(v));
+    ClipStats st;
+    const double r = clip_unit_cube<double>(v, &st);
+    if (faces_edges_vols) {
+        faces_edges_vols[0] = st.faces;
+        faces_edges_vols[1] = st.edge_points;
+        faces_edges_vols[2] = st.subtet_vols;
+    }
+    return r;
+}
+int oracle_intersection_test_f64(const double* t) {
+    double v[4][3];
+    std::memcpy(v, t, sizeof(v));
+    return int(intersection_test<double>(v));
+}
+double oracle_vol6_f64(const double* t) { return vol6<double>(t, t + 3, t + 6, t + 9); }
+void oracle_decomposition(int decomp, int32_t* out24) {
+    int c[6][4];
+    decomposition_corners(decomp, c);
+    for (int t = 0; t < 6; ++t)
+        for (int v = 0; v < 4; ++v) out24[4 * t + v] = c[t][v];
+}
+
+// ------------------------------------------------------------------ octree
+struct OracleOctree {
+    MetaCells mc;
+    OTree tree;
+};
+
+void* oracle_octree_build(const float* x, const float* y, const float* z, const int* np,
+                          const float* roi6, uint64_t max_tets_param, int max_level) {
+    auto* h = new OracleOctree();
+    const int64_t n = int64_t(np[0]) * np[1] * np[2];
+    h->mc = build_meta_cells(x, y, z, np);
+    const FBox box = lattice_bbox(x, y, z, n);
+    const uint64_t num_tets = 6ull * uint64_t(np[0] - 1) * uint64_t(np[1] - 1) * uint64_t(np[2] - 1);
+    const uint64_t max_tets = std::min(num_tets, max_tets_param);  // AdDepositWorkFlow.h:169
+    FBox roi;
+    if (roi6) {
+        roi.mn[0] = roi6[0]; roi.mx[0] = roi6[1]; roi.mn[1] = roi6[2];
+        roi.mx[1] = roi6[3]; roi.mn[2] = roi6[4]; roi.mx[2] = roi6[5];
+    }
+    h->tree = build_octree(h->mc, box, roi6 ? &roi : nullptr, max_tets, max_level);
+    return h;
+}
+void oracle_octree_free(void* h) { delete static_cast<OracleOctree*>(h); }
+int64_t oracle_octree_num_nodes(void* h) { return int64_t(static_cast<OracleOctree*>(h)->tree.nodes.size()); }
+int64_t oracle_octree_num_leaves(void* h) { return int64_t(static_cast<OracleOctree*>(h)->tree.leaves_dfs.size()); }
+int64_t oracle_octree_num_meta_cells(void* h) { return int64_t(static_cast<OracleOctree*>(h)->mc.bbox.size()); }
+void oracle_octree_bbox(void* h, float* out6) {
+    const FBox& b = static_cast<OracleOctree*>(h)->tree.root_box;
+    for (int a = 0; a < 3; ++a) { out6[2 * a] = b.mn[a]; out6[2 * a + 1] = b.mx[a]; }
+}
+void oracle_octree_nodes(void* hp, float* ll, float* ext, int32_t* level, int32_t* parent,
+                         int32_t* first_child, uint64_t* count, uint64_t* local_count) {
+    const auto& nodes = static_cast<OracleOctree*>(hp)->tree.nodes;
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        for (int a = 0; a < 3; ++a) ll[3 * i + a] = nodes[i].ll[a];
+        ext[i] = nodes[i].ext;
+        level[i] = nodes[i].level;
+        parent[i] = nodes[i].parent;
+        first_child[i] = nodes[i].first_child;
+        count[i] = nodes[i].count;
+        local_count[i] = nodes[i].local_count;
+    }
+}
+int64_t oracle_octree_leaves(void* hp, int32_t* dfs, int32_t* order, int64_t* mc_offsets, int64_t* mc_ids) {
+    const auto* h = static_cast<OracleOctree*>(hp);
+    int64_t total = 0;
+    for (size_t l = 0; l < h->tree.leaves_dfs.size(); ++l) {
+        const auto& ids = h->tree.nodes[h->tree.leaves_dfs[l]].meta;
+        if (mc_ids) {
+            dfs[l] = h->tree.leaves_dfs[l];
+            order[l] = h->tree.leaves_order[l];
+            mc_offsets[l] = total;
+            for (size_t m = 0; m < ids.size(); ++m) mc_ids[total + m] = ids[m];
+        }
+        total += int64_t(ids.size());
+    }
+    if (mc_ids) mc_offsets[h->tree.leaves_dfs.size()] = total;
+    return total;
+}
+void oracle_meta_cell_bboxes(void* hp, float* out) {
+    const auto& mc = static_cast<OracleOctree*>(hp)->mc;
+    for (size_t m = 0; m < mc.bbox.size(); ++m)
+        for (int a = 0; a < 3; ++a) { out[6 * m + 2 * a] = mc.bbox[m].mn[a]; out[6 * m + 2 * a + 1] = mc.bbox[m].mx[a]; }
+}
+uint64_t oracle_estimate(void* hp, const float* box6, int64_t m) {
+    FBox b;
+    for (int a = 0; a < 3; ++a) { b.mn[a] = box6[2 * a]; b.mx[a] = box6[2 * a + 1]; }
+    return estimate(static_cast<OracleOctree*>(hp)->mc, b, m);
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+"""ctypes bindings of the oracle (oracle/liboracle.so) and of the compiled reference harness
+(oracle/_ref/libadeco_ref.so).  Test infrastructure: used only as the checker."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+REF = os.path.join(ROOT, "oracle", "_ref", "libadeco_ref.so")
+
+vp = ctypes.c_void_p
+sz = ctypes.c_size_t
+i64 = ctypes.c_int64
+u64 = ctypes.c_uint64
+dbl = ctypes.c_double
+cint = ctypes.c_int
+
+_ORACLE_SIGS = {
+    "oracle_deposit_patches_f32": (cint, [sz, sz, vp, sz, cint, vp, cint, vp]),
+    "oracle_deposit_patches_f64": (cint, [sz, sz, vp, sz, cint, vp, cint, dbl, vp, vp]),
+    "oracle_deposit_lattice": (cint, [vp, vp, vp, vp, cint, vp, i64, vp, cint, dbl, cint, vp, vp, cint, vp]),
+    "oracle_deposit_soup_f64": (cint, [vp, i64, vp, cint, dbl, vp, vp]),
+    "oracle_clip_unit_cube_f32": (ctypes.c_float, [vp]),
+    "oracle_clip_unit_cube_f64": (dbl, [vp, vp]),
+    "oracle_intersection_test_f64": (cint, [vp]),
+    "oracle_vol6_f64": (dbl, [vp]),
+    "oracle_decomposition": (None, [cint, vp]),
+    "oracle_octree_build": (vp, [vp, vp, vp, vp, vp, u64, cint]),
+    "oracle_octree_free": (None, [vp]),
+    "oracle_octree_num_nodes": (i64, [vp]),
+    "oracle_octree_num_leaves": (i64, [vp]),
+    "oracle_octree_num_meta_cells": (i64, [vp]),
+    "oracle_octree_bbox": (None, [vp, vp]),
+    "oracle_octree_nodes": (None, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    "oracle_octree_leaves": (i64, [vp, vp, vp, vp, vp]),
+    "oracle_meta_cell_bboxes": (None, [vp, vp]),
+    "oracle_estimate": (u64, [vp, vp, i64]),
+}
+
+_REF_SIGS = {
+    "ref_last_error": (ctypes.c_char_p, []),
+    "ref_deposit_tets": (cint, [sz, sz, vp, sz, cint, vp, cint, vp]),
+    "ref_octree_build": (cint, [vp, vp, vp, cint, cint, cint, vp, u64, cint]),
+    "ref_octree_bbox": (None, [vp]),
+    "ref_octree_num_nodes": (i64, []),
+    "ref_octree_num_leaves": (i64, []),
+    "ref_octree_num_meta_cells": (i64, []),
+    "ref_octree_nodes": (None, [vp, vp, vp, vp, vp, vp, vp]),
+    "ref_octree_leaves": (i64, [vp, vp, vp, vp]),
+    "ref_meta_cell_bboxes": (None, [vp]),
+    "ref_estimate": (u64, [vp, i64]),
+    "ref_octree_deposit_leaf": (cint, [ctypes.c_int32, cint, vp, vp, vp, ctypes.c_char_p]),
+}
+
+_cache = {}
+
+
+def _bind(path, sigs):
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def load():
+    if "o" not in _cache:
+        if not os.path.exists(ORACLE):
+            raise FileNotFoundError(f"{ORACLE} missing: run `make -C oracle`")
+        _cache["o"] = _bind(ORACLE, _ORACLE_SIGS)
+    return _cache["o"]
+
+
+def load_ref():
+    if "r" not in _cache:
+        _cache["r"] = _bind(REF, _REF_SIGS) if os.path.exists(REF) else None
+    return _cache["r"]
+
+
+def P(a):
+    return None if a is None else vp(a.ctypes.data)
+
+
+def deposit_patches_f64(patches, bbox, dims, tet_mass=1.0, patch_offset=0, num_patches=None, stats=False):
+    o = load()
+    patches = np.ascontiguousarray(patches, np.float32)
+    bbox = np.ascontiguousarray(bbox, np.float32)
+    n = patches.size // 1536 if num_patches is None else num_patches
+    g = np.zeros(dims ** 3, np.float64)
+    st = np.zeros(16, np.uint64)
+    rc = o.oracle_deposit_patches_f64(patch_offset, n, P(patches), patches.size, 7, P(bbox), dims, tet_mass,
+                                      P(g), P(st))
+    assert rc == 0
+    return (g, st) if stats else g
+
+
+def deposit_patches_f32(patches, bbox, dims, patch_offset=0, num_patches=None):
+    o = load()
+    patches = np.ascontiguousarray(patches, np.float32)
+    bbox = np.ascontiguousarray(bbox, np.float32)
+    n = patches.size // 1536 if num_patches is None else num_patches
+    g = np.zeros(dims ** 3, np.float32)
+    assert o.oracle_deposit_patches_f32(patch_offset, n, P(patches), patches.size, 7, P(bbox), dims, P(g)) == 0
+    return g
+
+
+def deposit_lattice(lat, bbox, dims, decomp=0, tet_mass=1.0, tet_ids=None, precision=1, nthreads=1, stats=False):
+    """precision 0 = float-faithful (float grid), 1 = FP64, 2 = FP64 with op counting."""
+    o = load()
+    np_ = np.array(lat.dims, np.int32)
+    bbox = np.ascontiguousarray(bbox, np.float32)
+    ids = None if tet_ids is None else np.ascontiguousarray(tet_ids, np.int32)
+    n = lat.num_tets if ids is None else ids.size
+    gf = np.zeros(dims ** 3, np.float32) if precision == 0 else None
+    gd = np.zeros(dims ** 3, np.float64) if precision != 0 else None
+    st = np.zeros(16, np.uint64)
+    o.oracle_deposit_lattice(P(lat.x), P(lat.y), P(lat.z), P(np_), decomp, P(ids), n, P(bbox), dims, tet_mass,
+                             precision, P(gf), P(gd), nthreads, P(st))
+    g = gf if precision == 0 else gd
+    return (g, st) if stats else g
+
+
+def deposit_soup_f64(tets, bbox, dims, tet_mass=1.0):
+    o = load()
+    tets = np.ascontiguousarray(tets, np.float32).reshape(-1, 12)
+    bbox = np.ascontiguousarray(bbox, np.float32)
+    g = np.zeros(dims ** 3, np.float64)
+    o.oracle_deposit_soup_f64(P(tets), tets.shape[0], P(bbox), dims, tet_mass, P(g), None)
+    return g
+
+
+def ref_deposit_tets(patches, bbox, dims, patch_offset=0, num_patches=None):
+    r = load_ref()
+    patches = np.ascontiguousarray(patches, np.float32)
+    bbox = np.ascontiguousarray(bbox, np.float32)
+    n = patches.size // 1536 if num_patches is None else num_patches
+    g = np.zeros(dims ** 3, np.float32)
+    assert r.ref_deposit_tets(patch_offset, n, P(patches), patches.size, 7, P(bbox), dims, P(g)) == 0
+    return g
+
+
+def max_rel_err(a, b, tet_mass=1.0, floor_frac=1e-8):
+    """Per-cell |a - b| / max(|b|, floor), floor = floor_frac * tet_mass.
+
+    Cells whose oracle mass is below 1e-8 of one tet's mass are held to an absolute error of
+    1e-8 * tolerance * tet_mass (1e-14 m at the 1e-6 bar): there two FP64 evaluations of a sliver
+    already differ by cancellation at the 1e-16 absolute level.  Every other cell is compared
+    relatively."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    denom = np.maximum(np.abs(b), floor_frac * tet_mass)
+    return float(np.max(np.abs(a - b) / denom)) if a.size else 0.0

@@ -1,0 +1,129 @@
+"""Writes the golden fixtures of the compiled reference (run in the build container only).
+
+Requires oracle/_ref/libadeco_ref.so, built by oracle/build_ref.sh from the unmodified reference
+sources under /root/reference/src.  Only outputs are stored, as small .npz files next to this
+script; the reference itself never travels.  Inputs are regenerated from their seeds by
+adeco_amd.lattice, whose output hash is stored so generator drift is detected.
+
+  ref_deposit.npz      DepositWrapper::deposit_tets float grids (sha256 + samples + total)
+  ref_octree_<n>.npz   generate_tree node tables, counts, leaf lists, meta-cell AABBs
+  ref_leaves_c2.npz    per-leaf float grids (sha256 + write_patch mass) for leaves of the
+                       128^3 configuration, deposited as one worker thread does
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from adeco_amd import cubic_root_box  # noqa: E402
+from adeco_amd.lattice import make_lattice, meta_patches  # noqa: E402
+from tests import _oracle as O  # noqa: E402
+from tests._octree_io import ref_octree  # noqa: E402
+
+DEPOSIT_CASES = [(8, 1.0, 16, 1), (12, 3.0, 24, 2), (10, 2.0, 7, 5), (32, 1.0, 64, 1)]
+OCTREE_CASES = [(32, 3.0, 2, 3000, 4), (48, 4.0, 6, 4000, 5), (128, 3.0, 2, 50000, 6)]
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def lattice_sha(lat) -> str:
+    return sha(np.concatenate([lat.x, lat.y, lat.z]))
+
+
+def deposit_fixture():
+    rng = np.random.default_rng(0)
+    out = {"cases": np.array(DEPOSIT_CASES, np.float64), "lattice_sha": [], "grid_sha": [], "total": [],
+           "sample_idx": [], "sample_val": [], "ref_seconds": []}
+    for n, rms, dims, seed in DEPOSIT_CASES:
+        lat = make_lattice(n, rms, seed=seed)
+        pat = meta_patches(lat)
+        bbox = cubic_root_box(lat.bbox())
+        t = time.perf_counter()
+        g = O.ref_deposit_tets(pat, bbox, dims)
+        out["ref_seconds"].append(time.perf_counter() - t)
+        out["lattice_sha"].append(lattice_sha(lat))
+        out["grid_sha"].append(sha(g))
+        out["total"].append(float(g.sum(dtype=np.float64)))
+        idx = np.sort(rng.choice(g.size, size=64, replace=False))
+        nz = np.flatnonzero(g)
+        idx = np.unique(np.concatenate([idx, nz[:: max(1, nz.size // 64)][:64]]))[:128]
+        idx = np.pad(idx, (0, 128 - idx.size), mode="edge")
+        out["sample_idx"].append(idx)
+        out["sample_val"].append(g[idx])
+        print(f"deposit n={n} dims={dims}: total={g.sum():.6f} ref {out['ref_seconds'][-1]:.2f}s", flush=True)
+    np.savez_compressed(os.path.join(HERE, "ref_deposit.npz"), cases=out["cases"],
+                        lattice_sha=np.array(out["lattice_sha"]), grid_sha=np.array(out["grid_sha"]),
+                        total=np.array(out["total"]), sample_idx=np.array(out["sample_idx"]),
+                        sample_val=np.array(out["sample_val"], np.float32), ref_seconds=np.array(out["ref_seconds"]))
+
+
+def octree_fixture(n, rms, seed, max_tets, max_level):
+    lat = make_lattice(n, rms, seed=seed)
+    d = ref_octree(lat, max_tets, max_level)
+    np.savez_compressed(os.path.join(HERE, f"ref_octree_{n}.npz"), params=np.array([n, rms, seed, max_tets, max_level]),
+                        lattice_sha=lattice_sha(lat), **d)
+    print(f"octree n={n}: {d['ll'].shape[0]} nodes, {d['dfs'].size} leaves", flush=True)
+    return lat, d
+
+
+def leaves_fixture(lat, d, dims=16, count=12):
+    r = O.load_ref()
+    order = d["order"]
+    pick = list(order[:4]) + list(order[-2:]) + list(np.random.default_rng(1).choice(order[4:-2], count - 6, replace=False))
+    rows = {"node": [], "bbox": [], "grid_sha": [], "mass": [], "density_sha": [], "grid_total": [], "seconds": [],
+            "n_meta": []}
+    grid = np.zeros(dims ** 3, np.float32)
+    mass = np.zeros(1)
+    bbox = np.zeros(6, np.float32)
+    with tempfile.TemporaryDirectory() as tmp:
+        for node in pick:
+            t = time.perf_counter()
+            assert r.ref_octree_deposit_leaf(int(node), dims, O.P(grid), O.P(mass), O.P(bbox), tmp.encode()) == 0
+            rows["seconds"].append(time.perf_counter() - t)
+            lvl = int(d["level"][node])
+            ext = float(d["ext"][node])
+            delta = ext / dims
+            root_ll = d["ll"][0].astype(np.float64)
+            io = [int(np.rint((float(d["ll"][node][a]) - root_ll[a]) / delta)) for a in range(3)]
+            fname = os.path.join(tmp, f"node_{lvl}_{io[0]}_{io[1]}_{io[2]}")
+            dens = np.fromfile(fname, dtype=np.float32)
+            rows["node"].append(int(node))
+            rows["bbox"].append(bbox.copy())
+            rows["grid_sha"].append(sha(grid))
+            rows["mass"].append(float(mass[0]))
+            rows["density_sha"].append(sha(dens))
+            rows["grid_total"].append(float(grid.sum(dtype=np.float64)))
+            leaf = int(np.flatnonzero(d["dfs"] == node)[0])
+            rows["n_meta"].append(int(d["mc_offsets"][leaf + 1] - d["mc_offsets"][leaf]))
+            print(f"leaf node {node}: level {lvl} mass {mass[0]:.4f} meta {rows['n_meta'][-1]} "
+                  f"{rows['seconds'][-1]:.2f}s", flush=True)
+    np.savez_compressed(os.path.join(HERE, "ref_leaves_c2.npz"), dims=dims, lattice_sha=lattice_sha(lat),
+                        node=np.array(rows["node"]), bbox=np.array(rows["bbox"]), grid_sha=np.array(rows["grid_sha"]),
+                        mass=np.array(rows["mass"]), density_sha=np.array(rows["density_sha"]),
+                        grid_total=np.array(rows["grid_total"]), seconds=np.array(rows["seconds"]),
+                        n_meta=np.array(rows["n_meta"]))
+
+
+def main():
+    if O.load_ref() is None:
+        sys.exit("oracle/_ref/libadeco_ref.so is not built (run oracle/build_ref.sh)")
+    deposit_fixture()
+    for case in OCTREE_CASES:
+        lat, d = octree_fixture(*case)
+    leaves_fixture(lat, d)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,124 @@
+"""GPU parity of the deposit kernels against the FP64 oracle (tests/_oracle.py).
+
+Tolerances (stated per test): per-cell relative error <= 1e-6 against the FP64 oracle, computed
+as |gpu - oracle| / max(|oracle|, 1e-8 * tet_mass); total mass conserved to 1e-9 relative when
+every tet lies inside the grid.
+"""
+
+import numpy as np
+import pytest
+
+import adeco_amd as A
+from adeco_amd.lattice import make_lattice, meta_patches
+from tests import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-6
+MASS_REL = 1e-9
+
+
+def padded_box(bbox):
+    """A cube strictly containing bbox with integer corners and a power-of-two edge, so its three
+    cell sizes are identical in float (mass conservation then holds to rounding)."""
+    b = np.asarray(bbox, np.float64)
+    lo = np.floor(b[0::2]) - 1
+    ext = 2.0 ** np.ceil(np.log2(np.max(b[1::2] - lo) + 1))
+    out = np.empty(6, np.float32)
+    out[0::2] = lo
+    out[1::2] = lo + ext
+    return out
+
+
+@pytest.mark.parametrize("n,rms,dims", [(8, 1.0, 16), (12, 3.0, 24)])
+def test_boundary_deposit_tets_matches_fp64_oracle(n, rms, dims):
+    print(f"[boundary] n={n} rms={rms} dims={dims}", flush=True)
+    lat = make_lattice(n, rms, seed=1)
+    pat = meta_patches(lat)
+    bbox = A.cubic_root_box(lat.bbox())
+    data = np.zeros(dims ** 3, np.float32)
+    A.DepositWrapper.deposit_tets(True, 0, 0, pat.size // 1536, pat, pat.size, 7, bbox, dims, data)
+    want = O.deposit_patches_f64(pat, bbox, dims)
+    # the boundary returns float (the reference's grid type): one float rounding per cell
+    assert O.max_rel_err(data, want) <= REL
+    # accumulation contract: a second call adds
+    A.DepositWrapper.deposit_tets(True, 0, 0, pat.size // 1536, pat, pat.size, 7, bbox, dims, data)
+    assert O.max_rel_err(data, 2 * want) <= REL
+
+
+def test_boundary_patch_subrange_and_multipass():
+    print("[boundary] subrange + multipass", flush=True)
+    lat = make_lattice(16, 2.0, seed=7)
+    pat = meta_patches(lat)
+    npatch = pat.size // 1536
+    bbox = A.cubic_root_box(lat.bbox())
+    dims = 20
+    off, cnt = 2, npatch - 3
+    want = O.deposit_patches_f64(pat, bbox, dims, patch_offset=off, num_patches=cnt)
+    one = np.zeros(dims ** 3, np.float32)
+    A.DepositWrapper.deposit_tets(True, 0, off, cnt, pat, pat.size, 7, bbox, dims, one)
+    assert O.max_rel_err(one, want) <= REL
+    # a budget that holds the grid plus 2 patches forces several passes; result unchanged
+    A.set_memory_budget(dims ** 3 * 8 + 2 * 1536 * 4 + 16)
+    try:
+        multi = np.zeros(dims ** 3, np.float32)
+        A.DepositWrapper.deposit_tets(True, 0, off, cnt, pat, pat.size, 7, bbox, dims, multi)
+    finally:
+        A.set_memory_budget(0)
+    assert O.max_rel_err(multi, want) <= REL
+    # use_specific_gpu = False splits over all visible GPUs (one here)
+    spread = np.zeros(dims ** 3, np.float32)
+    A.DepositWrapper.deposit_tets(False, 0, off, cnt, pat, pat.size, 7, bbox, dims, spread)
+    assert O.max_rel_err(spread, want) <= REL
+
+
+def test_boundary_errors_on_gpu():
+    lat = make_lattice(8, 1.0, seed=1)
+    pat = meta_patches(lat)
+    bbox = A.cubic_root_box(lat.bbox())
+    data = np.zeros(8 ** 3, np.float32)
+    with pytest.raises(A.AdRuntimeException):
+        A.DepositWrapper.deposit_tets(True, 0, 1, pat.size // 1536, pat, pat.size, 7, bbox, 8, data)
+    with pytest.raises(A.AdRuntimeException):
+        A.DepositWrapper.deposit_tets(True, 99, 0, 1, pat, pat.size, 7, bbox, 8, data)
+    # empty positions: no-op (AdDeposit2Grid.cpp:86-87)
+    A.DepositWrapper.deposit_tets(True, 0, 0, 0, pat, 0, 7, bbox, 8, data)
+    assert not data.any()
+
+
+@pytest.mark.parametrize("decomp", [A.DECOMP_REF_EFFECTIVE, A.DECOMP_REF_INTENDED, A.DECOMP_KUHN])
+@pytest.mark.parametrize("n,rms,dims", [(10, 1.0, 8), (10, 3.0, 32), (8, 0.3, 4)])
+def test_lattice_plain_matches_fp64_oracle(decomp, n, rms, dims):
+    print(f"[lattice plain] decomp={decomp} n={n} rms={rms} dims={dims}", flush=True)
+    lat = make_lattice(n, rms, seed=3)
+    box = padded_box(lat.bbox())
+    with A.ResidentLattice.from_lattice(lat) as rl:
+        got, st = rl.deposit_grid(box, dims, decomposition=decomp, tet_mass=1.0 / 6, stats=True)
+    want = O.deposit_lattice(lat, box, dims, decomp=decomp, tet_mass=1.0 / 6)
+    assert O.max_rel_err(got, want, tet_mass=1.0 / 6) <= REL
+    total = lat.n ** 3  # 6 n^3 tets of mass 1/6, all inside the padded box
+    assert abs(got.sum() - total) <= MASS_REL * total
+    assert int(st[0]) == lat.num_tets
+
+
+def test_lattice_subset_and_bbox():
+    lat = make_lattice(9, 2.0, seed=11)
+    with A.ResidentLattice.from_lattice(lat) as rl:
+        assert np.array_equal(rl.bbox(), lat.bbox())
+        ids = np.arange(0, lat.num_tets, 7, dtype=np.int32)
+        box = A.cubic_root_box(lat.bbox())
+        got = rl.deposit_grid(box, 16, tet_ids=ids)
+    want = O.deposit_lattice(lat, box, 16, tet_ids=ids)
+    assert O.max_rel_err(got, want) <= REL
+
+
+def test_classification_regimes():
+    """Tets much smaller than cells (tet-in-cell) and much larger (cell-in-tet) both match."""
+    lat = make_lattice(6, 0.5, seed=5)
+    box = padded_box(lat.bbox())
+    with A.ResidentLattice.from_lattice(lat) as rl:
+        coarse, st_c = rl.deposit_grid(box, 2, decomposition=A.DECOMP_KUHN, stats=True)
+        fine, st_f = rl.deposit_grid(box, 96, decomposition=A.DECOMP_KUHN, stats=True)
+    assert st_c[5] > 0 and st_f[6] > 0
+    assert O.max_rel_err(coarse, O.deposit_lattice(lat, box, 2, decomp=2)) <= REL
+    assert O.max_rel_err(fine, O.deposit_lattice(lat, box, 96, decomp=2)) <= REL

@@ -1,0 +1,204 @@
+"""Pins the oracle (oracle/liboracle.so) before it is trusted as the checker.
+
+* Float-faithful mode vs the compiled reference CPU path (oracle/_ref, built from
+  /root/reference/src where present) — bit-exact grids — and vs the committed golden fixtures
+  written by tests/golden/make_golden.py (sha256 of the reference's float grid bytes).
+* The clip in FP64 and float vs the Irwin–Hall closed form, an independent exact rational clipper
+  (tests/rational_clip.py), and invariants that a dropped term, a wrong sign or a transposed
+  operand would break: translation partition of unity, containment, axis permutation and
+  reflection symmetry.
+* The classification vs the clip: no-intersection -> 0, cell-inside-tet -> 1, tet-inside-cell -> |T|.
+"""
+
+import hashlib
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import adeco_amd as A
+from adeco_amd.lattice import make_lattice, meta_patches
+from tests import _oracle as O
+from tests.rational_clip import tet_box_volume
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def clip64(t):
+    t = np.ascontiguousarray(t, np.float64).reshape(4, 3)
+    return O.load().oracle_clip_unit_cube_f64(O.P(t), None)
+
+
+def clip32(t):
+    t = np.ascontiguousarray(t, np.float32).reshape(4, 3)
+    return O.load().oracle_clip_unit_cube_f32(O.P(t))
+
+
+def vol6(t):
+    t = np.ascontiguousarray(t, np.float64).reshape(4, 3)
+    return O.load().oracle_vol6_f64(O.P(t))
+
+
+def oriented(t):
+    """Vertex order with positive vol6 in the reference's convention (clip precondition)."""
+    t = np.array(t, np.float64).reshape(4, 3)
+    return t[[1, 0, 2, 3]] if vol6(t) < 0 else t
+
+
+def irwin_hall(s):
+    if s >= 3:
+        return 1.0
+    return sum((-1) ** k * math.comb(3, k) * max(s - k, 0.0) ** 3 for k in range(4)) / 6.0
+
+
+# ---------------------------------------------------------------- reference bit-exactness
+REF_CASES = [(8, 1.0, 16, 1), (12, 3.0, 24, 2), (10, 2.0, 7, 5)]
+
+
+@pytest.mark.parametrize("n,rms,dims,seed", REF_CASES)
+def test_float_faithful_oracle_is_bit_exact_vs_reference(ref, n, rms, dims, seed):
+    lat = make_lattice(n, rms, seed=seed)
+    pat = meta_patches(lat)
+    bbox = A.cubic_root_box(lat.bbox())
+    want = O.ref_deposit_tets(pat, bbox, dims)
+    got = O.deposit_patches_f32(pat, bbox, dims)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # a patch sub-range (patch_offset, num_patches) as the reference's worker chunks use
+    k = pat.size // 1536
+    want2 = O.ref_deposit_tets(pat, bbox, dims, patch_offset=1, num_patches=k - 2)
+    got2 = O.deposit_patches_f32(pat, bbox, dims, patch_offset=1, num_patches=k - 2)
+    assert np.array_equal(got2.view(np.uint32), want2.view(np.uint32))
+
+
+def _golden(name):
+    path = os.path.join(GOLDEN, name)
+    if not os.path.exists(path):
+        pytest.skip(f"{name} not generated")
+    return np.load(path, allow_pickle=False)
+
+
+def test_float_faithful_oracle_matches_golden_reference_grids():
+    g = _golden("ref_deposit.npz")
+    for i, (n, rms, dims, seed) in enumerate(g["cases"]):
+        lat = make_lattice(int(n), float(rms), seed=int(seed))
+        assert hashlib.sha256(np.concatenate([lat.x, lat.y, lat.z]).tobytes()).hexdigest() == str(g["lattice_sha"][i])
+        pat = meta_patches(lat)
+        bbox = A.cubic_root_box(lat.bbox())
+        got = O.deposit_patches_f32(pat, bbox, int(dims))
+        assert hashlib.sha256(got.tobytes()).hexdigest() == str(g["grid_sha"][i])
+        idx = g["sample_idx"][i]
+        assert np.array_equal(got[idx], g["sample_val"][i])
+
+
+def test_fp64_oracle_vs_reference_float_error_model(ref):
+    """The reference's float grid differs from FP64 by rounding only: bounded by
+    c * eps32 * (|x|/dx + sqrt(contributions)) per cell; here a loose 2e-4 relative on cells
+    above 1e-3 of a tet mass, and total mass to 1e-6."""
+    lat = make_lattice(10, 1.5, seed=9)
+    pat = meta_patches(lat)
+    bbox = A.cubic_root_box(lat.bbox())
+    f = O.ref_deposit_tets(pat, bbox, 16).astype(np.float64)
+    d = O.deposit_patches_f64(pat, bbox, 16)
+    big = d > 1e-3
+    assert np.max(np.abs(f[big] - d[big]) / d[big]) < 2e-4
+    assert abs(f.sum() - d.sum()) / d.sum() < 1e-6
+
+
+# ---------------------------------------------------------------- clip closed forms
+@pytest.mark.parametrize("s", [0.05, 0.5, 0.999, 1.0, 1.25, 2.0, 2.5, 2.99, 3.0, 4.0])
+def test_clip_irwin_hall(s):
+    # T_s = conv{0, s e1, s e2, s e3}; order (0, s e2, s e1, s e3) has positive vol6
+    t = [[0, 0, 0], [0, s, 0], [s, 0, 0], [0, 0, s]]
+    assert clip64(t) == pytest.approx(irwin_hall(s), rel=1e-14, abs=1e-16)
+    assert clip32(t) == pytest.approx(irwin_hall(s), rel=2e-6, abs=1e-7)
+
+
+def test_clip_against_exact_rational_clipper():
+    rng = np.random.default_rng(12)
+    for _ in range(60):
+        v = rng.integers(-6, 14, size=(4, 3)) / 8.0  # dyadic: exact in binary and in Fraction
+        if vol6(v) == 0:
+            continue
+        v = oriented(v)
+        exact = tet_box_volume([[Fraction(x) for x in p] for p in v.tolist()])
+        assert clip64(v) == pytest.approx(float(exact), rel=1e-12, abs=1e-15)
+
+
+def test_clip_invariants():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        t = oriented(rng.uniform(-0.7, 1.7, size=(4, 3)))
+        c = clip64(t)
+        # axis permutation (x<->y swaps orientation -> reorder) and reflection x -> 1 - x
+        p = oriented(t[:, [1, 0, 2]])
+        assert clip64(p) == pytest.approx(c, rel=1e-12, abs=1e-14)
+        r = t.copy()
+        r[:, 2] = 1.0 - r[:, 2]
+        assert clip64(oriented(r)) == pytest.approx(c, rel=1e-12, abs=1e-14)
+    # partition of unity: sum over integer translates of the cell = |T|
+    for _ in range(30):
+        t = oriented(rng.uniform(-1.3, 2.4, size=(4, 3)))
+        total = 0.0
+        for dx in range(-2, 3):
+            for dy in range(-2, 3):
+                for dz in range(-2, 3):
+                    total += clip64(t - np.array([dx, dy, dz], float))
+        assert total == pytest.approx(abs(vol6(t)) / 6, rel=1e-12)
+    # containment
+    small = oriented(0.3 + 0.2 * rng.uniform(size=(4, 3)))
+    assert clip64(small) == pytest.approx(abs(vol6(small)) / 6, rel=1e-13)
+    big = oriented([[-1, -1, -1], [9, -1, -1], [-1, 9, -1], [-1, -1, 9]])
+    assert clip64(big) == pytest.approx(1.0, rel=1e-14)
+
+
+def test_classification_agrees_with_clip():
+    o = O.load()
+    rng = np.random.default_rng(21)
+    seen = set()
+    for _ in range(3000):
+        scale = 10 ** rng.uniform(-1.5, 1.0)
+        t = oriented(rng.uniform(-1, 2, 3) + scale * rng.standard_normal((4, 3)))
+        tt = np.ascontiguousarray(t)
+        cls = o.oracle_intersection_test_f64(O.P(tt))
+        c = clip64(t)
+        seen.add(cls)
+        if cls == 0:
+            assert c == pytest.approx(0.0, abs=1e-12)
+        elif cls == 1:
+            assert c == pytest.approx(abs(vol6(t)) / 6, rel=1e-12)
+        elif cls == 2:
+            assert c == pytest.approx(1.0, rel=1e-12)
+    assert seen == {0, 1, 2, 3}
+
+
+# ---------------------------------------------------------------- per-tet deposit invariants
+def test_fp64_oracle_mass_conservation_and_decompositions():
+    lat = make_lattice(6, 0.8, seed=4)
+    # integer corners and a power-of-two extent: the three cell sizes are identical in float
+    # (the reference scales every axis by 1/dx, AdDepositTets2Grid.h:569)
+    box = np.array([-8, 120, -8, 120, -8, 120], np.float32)
+    for decomp in (0, 1, 2):
+        g = O.deposit_lattice(lat, box, 12, decomp=decomp)
+        assert g.sum() == pytest.approx(6 * lat.n ** 3, rel=1e-12)
+    # REF_INTENDED and KUHN tile each cube: on an undisplaced lattice both give the same density
+    und = make_lattice(4, 0.0)
+    box = np.array([-14, 114, -14, 114, -14, 114], np.float32)
+    gi = O.deposit_lattice(und, box, 11, decomp=1)
+    gk = O.deposit_lattice(und, box, 11, decomp=2)
+    assert np.allclose(gi, gk, rtol=1e-12, atol=1e-12)
+
+
+def test_effective_decomposition_signs():
+    """On an undisplaced lattice the reference's effective tets have signed vol6 (+,+,+,-,-,-)
+    (cube side 1), and REF_INTENDED tets all have vol6 = -1."""
+    o = O.load()
+    for decomp, want in ((0, [1, 1, 1, -1, -1, -1]), (1, [-1] * 6)):
+        c = np.zeros(24, np.int32)
+        o.oracle_decomposition(decomp, O.P(c))
+        got = []
+        for t in range(6):
+            pts = np.array([[(k & 1), (k >> 1) & 1, (k >> 2) & 1] for k in c[4 * t:4 * t + 4]], float)
+            got.append(round(vol6(pts)))
+        assert got == want
