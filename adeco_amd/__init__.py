@@ -64,6 +64,20 @@ def last_kernel_ms() -> float:
     return load().adeco_last_kernel_ms()
 
 
+SCALING_EXACT = 0
+SCALING_REFERENCE = 1
+
+
+def set_cell_scaling(mode: int):
+    """SCALING_EXACT (default): per-axis cell scaling, exact partition and mass conservation;
+    SCALING_REFERENCE: the reference's 1/dx on every axis (AdDepositTets2Grid.h:569)."""
+    _check(load().adeco_set_cell_scaling(int(mode)), "set_cell_scaling()")
+
+
+def get_cell_scaling() -> int:
+    return load().adeco_get_cell_scaling()
+
+
 def set_memory_budget(nbytes: int):
     """Per-call device memory budget (0 = 75% of device memory); lower values force multi-pass."""
     _check(load().adeco_set_memory_budget(int(nbytes)), "set_memory_budget()")
@@ -233,6 +247,22 @@ class Octree:
         out[:, 1::2] = (nd["ll"][dfs] + nd["ext"][dfs][:, None]).astype(np.float32)
         return out
 
+    def leaf_boxes_f64(self, exact: bool = True) -> np.ndarray:
+        """Leaf boxes [num_leaves, 6] (x0,x1,y0,y1,z0,z1) as the deposit uses them: the exact dyadic
+        FP64 subdivision of the root (SCALING_EXACT) or the float reference boxes."""
+        out = np.zeros((self.num_leaves, 6), np.float64)
+        _check(load().adeco_octree_leaf_boxes(self.handle, int(bool(exact)), ptr(out)), "octree_leaf_boxes()")
+        return out
+
+    def root_box_f64(self) -> np.ndarray:
+        """The root box (ll, ll + ext) in FP64 (the region the exact leaves tile)."""
+        nd = self.nodes()
+        ll, ext = nd["ll"][0].astype(np.float64), float(nd["ext"][0])
+        out = np.empty(6)
+        out[0::2] = ll
+        out[1::2] = ll + ext
+        return out
+
     def bin(self, decomposition: int = DECOMP_REF_EFFECTIVE) -> np.ndarray:
         counts = np.zeros(self.num_leaves, np.int64)
         _check(load().adeco_octree_bin(self.handle, int(decomposition), ptr(counts)), "octree_bin()")
@@ -245,6 +275,23 @@ class Octree:
                                            int(precision)), "deposit_leaves()")
         self.dims = int(dims)
         return last_kernel_ms()
+
+    def leaf_tets(self, leaf: int) -> np.ndarray:
+        """Binned lattice tet ids of one leaf (DFS index)."""
+        lib = load()
+        n = lib.adeco_octree_leaf_tets(self.handle, int(leaf), None)
+        if n < 0:
+            raise AdRuntimeException(ADECO_EINVAL, "ERROR: leaf_tets(): not binned or bad leaf")
+        out = np.zeros(max(n, 1), np.int32)
+        lib.adeco_octree_leaf_tets(self.handle, int(leaf), ptr(out))
+        return out[:n]
+
+    @staticmethod
+    def last_timings() -> dict:
+        out = np.zeros(5, np.float64)
+        load().adeco_last_timings(ptr(out))
+        return {"total_ms": out[0], "main_ms": out[1], "candidates": int(out[2]), "partials": int(out[3]),
+                "fallbacks": int(out[4])}
 
     def leaf_grids(self, first: int = 0, count=None) -> np.ndarray:
         """FP64 mass grids [count, dims^3] of leaves first..first+count (DFS order)."""

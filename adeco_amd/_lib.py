@@ -24,6 +24,8 @@ SIGNATURES = {
     "adeco_last_error": (ctypes.c_char_p, []),
     "adeco_version": (ctypes.c_int, []),
     "adeco_device_count": (ctypes.c_int, []),
+    "adeco_set_cell_scaling": (ctypes.c_int, [ctypes.c_int]),
+    "adeco_get_cell_scaling": (ctypes.c_int, []),
     "adeco_deposit_patches": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
                                              c_vp, ctypes.c_size_t, ctypes.c_int, c_vp, ctypes.c_int, c_vp]),
     "adeco_get_grid_size": (ctypes.c_int, []),
@@ -51,6 +53,9 @@ SIGNATURES = {
     "adeco_deposit_leaves": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                             ctypes.c_int]),
     "adeco_leaf_grids": (ctypes.c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_vp]),
+    "adeco_octree_leaf_tets": (ctypes.c_int64, [c_vp, ctypes.c_int64, c_vp]),
+    "adeco_octree_leaf_boxes": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp]),
+    "adeco_last_timings": (None, [c_vp]),
     "adeco_selftest_clip_vol6_f64": (ctypes.c_double, [c_vp]),
     "adeco_selftest_clip_vol6_f32": (ctypes.c_float, [c_vp]),
     "adeco_selftest_classify_f64": (ctypes.c_int, [c_vp]),

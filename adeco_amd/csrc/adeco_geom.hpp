@@ -46,6 +46,16 @@ struct CornerTable {
     unsigned char c[6][4];
 };
 
+// The same table packed into two 64-bit words (16 bits per tet, 3 bits per corner), so a kernel
+// reads corner codes with shifts instead of indexing a private array.
+struct PackedCorners {
+    unsigned long long lo, hi;  // tets 0-3, tets 4-5
+    ADECO_HD int code(int lt, int q) const {
+        const unsigned long long w = lt < 4 ? (lo >> (16 * lt)) : (hi >> (16 * (lt - 4)));
+        return int((w >> (3 * q)) & 7u);
+    }
+};
+
 ADECO_HD CornerTable corner_table(int decomp) {
     CornerTable t;
     // slot -> corner code for the two reference slot orders
@@ -66,6 +76,17 @@ ADECO_HD CornerTable corner_table(int decomp) {
         }
     }
     return t;
+}
+
+inline PackedCorners pack_corners(const CornerTable& t) {
+    PackedCorners p{0ull, 0ull};
+    for (int i = 0; i < 6; ++i)
+        for (int v = 0; v < 4; ++v) {
+            const unsigned long long bits = (unsigned long long)(t.c[i][v] & 7u) << (3 * v);
+            if (i < 4) p.lo |= bits << (16 * i);
+            else p.hi |= bits << (16 * (i - 4));
+        }
+    return p;
 }
 
 template <class R>
@@ -170,25 +191,33 @@ ADECO_HD void clip_stage(const V3<R> t[4], int axis, R c, bool ge, Stage<R>& st)
     }
     const int m = int(d[0] > R(0)) | (int(d[1] > R(0)) << 1) | (int(d[2] > R(0)) << 2) | (int(d[3] > R(0)) << 3);
     const int n = (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1) + ((m >> 3) & 1);
-    if (n == 0) { st.count = 0; return; }
-    if (n == 4) {
-        st.s[0] = t[0]; st.s[1] = t[1]; st.s[2] = t[2]; st.s[3] = t[3];
-        st.count = 1;
+    // every path writes all six slots, so the sequence stays in registers
+    if (n == 0 || n == 4) {
+        st.s[0] = t[0]; st.s[1] = t[1]; st.s[2] = t[2]; st.s[3] = t[3]; st.s[4] = t[3]; st.s[5] = t[3];
+        st.count = n == 4 ? 1 : 0;
         return;
     }
-    // canonical order: inside vertices first
-    int ord[4];
-    int k = 0;
-    for (int i = 0; i < 4; ++i) if (m & (1 << i)) ord[k++] = i;
-    for (int i = 0; i < 4; ++i) if (!(m & (1 << i))) ord[k++] = i;
-    V3<R> p[4];
-    R dd[4];
-    for (int i = 0; i < 4; ++i) { p[i] = t[ord[i]]; dd[i] = d[ord[i]]; }
+    // canonical order, inside vertices first: a compare-and-swap network on the inside flag keeps
+    // every access statically indexed (registers, no private-memory arrays)
+    V3<R> p[4] = {t[0], t[1], t[2], t[3]};
+    R dd[4] = {d[0], d[1], d[2], d[3]};
+#define ADECO_CSWAP(a, b)                                        \
+    {                                                            \
+        const bool sw = !(dd[a] > R(0)) && (dd[b] > R(0));       \
+        const V3<R> pa = p[a], pb = p[b];                        \
+        const R da = dd[a], db = dd[b];                          \
+        p[a] = sel(sw, pb, pa); p[b] = sel(sw, pa, pb);          \
+        dd[a] = sw ? db : da; dd[b] = sw ? da : db;              \
+    }
+    ADECO_CSWAP(0, 1) ADECO_CSWAP(2, 3) ADECO_CSWAP(0, 2) ADECO_CSWAP(1, 3) ADECO_CSWAP(1, 2)
+#undef ADECO_CSWAP
     if (n == 1) {
         st.s[0] = p[0];
         st.s[1] = cut_point(p[0], dd[0], p[1], dd[1], axis, c);
         st.s[2] = cut_point(p[0], dd[0], p[2], dd[2], axis, c);
         st.s[3] = cut_point(p[0], dd[0], p[3], dd[3], axis, c);
+        st.s[4] = st.s[3];
+        st.s[5] = st.s[3];
         st.count = 1;
     } else if (n == 3) {  // prism (a,b,c) - (p_ao, p_bo, p_co)
         st.s[0] = p[0]; st.s[1] = p[1]; st.s[2] = p[2];
@@ -207,9 +236,15 @@ ADECO_HD void clip_stage(const V3<R> t[4], int axis, R c, bool ge, Stage<R>& st)
     }
 }
 
+// sub-tet k (0..2) = seq[k..k+3]; written out slot by slot so every index is a constant and the
+// sequence never leaves registers
 template <class R>
 ADECO_HD void take(const Stage<R>& st, int k, V3<R> out[4]) {
-    for (int j = 0; j < 4; ++j) out[j] = sel(k == 0, st.s[j], sel(k == 1, st.s[j + 1], st.s[j + 2]));
+    const bool k0 = k == 0, k1 = k == 1;
+    out[0] = sel(k0, st.s[0], sel(k1, st.s[1], st.s[2]));
+    out[1] = sel(k0, st.s[1], sel(k1, st.s[2], st.s[3]));
+    out[2] = sel(k0, st.s[2], sel(k1, st.s[3], st.s[4]));
+    out[3] = sel(k0, st.s[3], sel(k1, st.s[4], st.s[5]));
 }
 
 // Which of the faces x>=0, x<=1, y>=0, y<=1 cut the tet (bit f set), or -1 if the tet lies

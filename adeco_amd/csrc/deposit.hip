@@ -31,7 +31,7 @@ template <bool kCount>
 __global__ void __launch_bounds__(256) k_deposit_patches_plain(const float* __restrict__ pos,
                                                                int64_t n_tets, int m, GridGeom g,
                                                                double tet_mass, double* grid,
-                                                               unsigned long long* d_stats) {
+                                                               unsigned long long* d_stats, PackedCorners pc) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n_tets) return;
     const int64_t cell = t / 6;
@@ -47,10 +47,9 @@ __global__ void __launch_bounds__(256) k_deposit_patches_plain(const float* __re
     const int slot_off[8] = {0, ax, ay, ay + ax, ay + az, ay + az + ax, az, az + ax};
     for (int s = 0; s < 8; ++s)
         if (is_sentinel(base[slot_off[s]], base[slot_off[s] + 1], base[slot_off[s] + 2])) return;
-    const CornerTable ct = corner_table(REF_EFFECTIVE);
     V3<double> v[4];
     for (int q = 0; q < 4; ++q) {
-        const int code = ct.c[lt][q];
+        const int code = pc.code(lt, q);
         const float* pv = base + 3 * (code & 1) + ay * ((code >> 1) & 1) + az * ((code >> 2) & 1);
         v[q] = mk(double(pv[0]), double(pv[1]), double(pv[2]));
     }
@@ -65,7 +64,7 @@ __global__ void __launch_bounds__(256) k_deposit_patches_plain(const float* __re
 }
 
 template <bool kCount>
-__global__ void __launch_bounds__(256) k_deposit_lattice_plain(LatticeView lat, CornerTable ct,
+__global__ void __launch_bounds__(256) k_deposit_lattice_plain(LatticeView lat, PackedCorners ct,
                                                                const int32_t* __restrict__ tet_ids,
                                                                int64_t n_tets, GridGeom g, double tet_mass,
                                                                double* grid, unsigned long long* d_stats) {
@@ -122,12 +121,13 @@ hipError_t launch_deposit_patches_plain(const float* d_positions, int64_t num_pa
                                         unsigned long long* d_stats, hipStream_t stream) {
     const int64_t n = num_patches * int64_t(m) * m * m * 6;
     if (n <= 0) return hipSuccess;
+    const PackedCorners pc = pack_corners(corner_table(REF_EFFECTIVE));
     if (d_stats)
         hipLaunchKernelGGL(k_deposit_patches_plain<true>, dim3(blocks_for(n, 256)), dim3(256), 0, stream,
-                           d_positions, n, m, g, tet_mass, d_grid, d_stats);
+                           d_positions, n, m, g, tet_mass, d_grid, d_stats, pc);
     else
         hipLaunchKernelGGL(k_deposit_patches_plain<false>, dim3(blocks_for(n, 256)), dim3(256), 0, stream,
-                           d_positions, n, m, g, tet_mass, d_grid, d_stats);
+                           d_positions, n, m, g, tet_mass, d_grid, d_stats, pc);
     return hipGetLastError();
 }
 
@@ -135,7 +135,7 @@ hipError_t launch_deposit_lattice_plain(const LatticeView& lat, int decomp, cons
                                         int64_t n_tets, const GridGeom& g, double tet_mass,
                                         double* d_grid, unsigned long long* d_stats, hipStream_t stream) {
     if (n_tets <= 0) return hipSuccess;
-    const CornerTable ct = corner_table(decomp);
+    const PackedCorners ct = pack_corners(corner_table(decomp));
     if (d_stats)
         hipLaunchKernelGGL(k_deposit_lattice_plain<true>, dim3(blocks_for(n_tets, 256)), dim3(256), 0, stream,
                            lat, ct, d_tet_ids, n_tets, g, tet_mass, d_grid, d_stats);

@@ -3,7 +3,7 @@
 // Follows deposit_tet_2_grid (AdDepositTets2Grid.h:509-665): tet AABB and its strict overlap with
 // the grid, the covered cell range [floor, ceil] clamped to the grid, the signed volume (zero ->
 // skip; negative -> vertex order (v3, v1, v2, v0)), then per covered cell the strict AABB cull,
-// the scaled tet s = scale * (src - cell_min), the classification and the mass
+// the scaled tet s = scale_a * (src - cell_min) per axis, the classification and the mass
 //   tet in cell  -> m
 //   cell in tet  -> m * cell_vol / |V|
 //   partial      -> m * (|s ∩ [0,1]^3| * cell_vol) / |V|      (when > 0)
@@ -69,7 +69,7 @@ ADECO_HD void deposit_tet_f64(const V3<double> v[4], const GridGeom& g, double t
                 if (cnt) cnt->v[3]++;
                 V3<double> s[4];
                 for (int q = 0; q < 4; ++q)
-                    s[q] = mk(g.scale * (src[q].x - cx), g.scale * (src[q].y - cy), g.scale * (src[q].z - cz));
+                    s[q] = mk(g.scale[0] * (src[q].x - cx), g.scale[1] * (src[q].y - cy), g.scale[2] * (src[q].z - cz));
                 const int cls = classify_unit(s);
                 double dm = 0.0;
                 if (cls == CLS_NONE) {
@@ -103,7 +103,7 @@ ADECO_HD bool is_sentinel(float x, float y, float z) {  // AdDepositTets2Grid.h:
 }
 
 // The 4 vertices of lattice tet t (cube t/6, i fastest over (np-1)^3 cubes; local tet t%6).
-ADECO_HD void lattice_tet_vertices(const LatticeView& lat, const CornerTable& ct, int64_t t, V3<double> v[4]) {
+ADECO_HD void lattice_tet_vertices(const LatticeView& lat, const PackedCorners& ct, int64_t t, V3<double> v[4]) {
     const int64_t c = t / 6;
     const int lt = int(t - 6 * c);
     const int64_t ncx = lat.np[0] - 1, ncy = lat.np[1] - 1;
@@ -112,7 +112,7 @@ ADECO_HD void lattice_tet_vertices(const LatticeView& lat, const CornerTable& ct
     const int64_t k = cj / ncy;
     const int64_t j = cj - k * ncy;
     for (int q = 0; q < 4; ++q) {
-        const int code = ct.c[lt][q];
+        const int code = ct.code(lt, q);
         const int64_t vi = (i + (code & 1)) +
                            int64_t(lat.np[0]) * ((j + ((code >> 1) & 1)) + int64_t(lat.np[1]) * (k + ((code >> 2) & 1)));
         v[q] = mk(double(lat.x[vi]), double(lat.y[vi]), double(lat.z[vi]));

@@ -12,28 +12,43 @@
 namespace adeco {
 
 // One uniform output grid (an octree leaf, a request ROI or the boundary call's bbox).
-// Built on the host from the caller's float bbox exactly as the reference does in FP32, but in
-// FP64: delta = (max - min) / dims, cell_vol = dx*dy*dz, scale = 1/dx (AdDepositTets2Grid.h:527-569).
+// Built on the host from the caller's float bbox as the reference does in FP32, but in FP64:
+// delta = (max - min) / dims, cell_vol = dx*dy*dz (AdDepositTets2Grid.h:527-569).  The reference
+// then scales every axis by 1/dx (AdDepositTets2Grid.h:569), which assumes cubic cells; a float
+// box (ll, ll + ext) rarely has three bit-identical extents, so that formula moves the cell faces
+// by up to ~1e-7 of a cell and breaks mass conservation at that level.  kScalingExact (the
+// default) scales each axis by its own 1/delta, which partitions the box exactly;
+// kScalingReference keeps the reference's formula.
+enum CellScaling : int { kScalingExact = 0, kScalingReference = 1 };
+
 struct GridGeom {
     double gmin[3], gmax[3];
     double delta[3];
-    double scale;
+    double scale[3];
     double cell_vol;
     int dims;
 };
 
-inline GridGeom make_grid_geom(const float bbox[6], int dims) {
+inline GridGeom make_grid_geom_d(const double bbox[6], int dims, int scaling = kScalingExact) {
     GridGeom g;
     for (int a = 0; a < 3; ++a) {
-        g.gmin[a] = double(bbox[2 * a]);
-        g.gmax[a] = double(bbox[2 * a + 1]);
+        g.gmin[a] = bbox[2 * a];
+        g.gmax[a] = bbox[2 * a + 1];
         g.delta[a] = (g.gmax[a] - g.gmin[a]) / double(dims);
     }
     g.cell_vol = g.delta[0] * g.delta[1] * g.delta[2];
-    g.scale = 1.0 / g.delta[0];
+    for (int a = 0; a < 3; ++a) g.scale[a] = 1.0 / g.delta[scaling == kScalingReference ? 0 : a];
     g.dims = dims;
     return g;
 }
+
+inline GridGeom make_grid_geom(const float bbox[6], int dims, int scaling = kScalingExact) {
+    const double b[6] = {bbox[0], bbox[1], bbox[2], bbox[3], bbox[4], bbox[5]};
+    return make_grid_geom_d(b, dims, scaling);
+}
+
+// Library-wide cell scaling (adeco_set_cell_scaling); read when a deposit starts.
+int cell_scaling();
 
 // Resident lattice: SoA float vertex coordinates, np[0]*np[1]*np[2] vertices, i fastest.
 struct LatticeView {

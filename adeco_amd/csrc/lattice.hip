@@ -161,7 +161,7 @@ adeco_status adeco_lattice_deposit_grid(adeco_lattice L, int decomposition, doub
     if (st) return st;
     if (tet_ids && (st = ds.pos.reserve(size_t(n_tets) * sizeof(int32_t) + 4))) return st;
     if ((st = ds.stats.reserve(kNumStats * sizeof(unsigned long long)))) return st;
-    const GridGeom g = make_grid_geom(bbox6, dims);
+    const GridGeom g = make_grid_geom(bbox6, dims, cell_scaling());
     e = launch_fill_f64(ds.grid.as<double>(), int64_t(ncell), 0.0, ds.stream);
     if (e == hipSuccess && tet_ids)
         e = hipMemcpyAsync(ds.pos.ptr, tet_ids, size_t(n_tets) * sizeof(int32_t), hipMemcpyHostToDevice, ds.stream);
@@ -199,7 +199,7 @@ adeco_status adeco_deposit_soup(int device_id, const float* tets, int64_t n_tets
     if ((st = ds.grid.reserve(ncell * sizeof(double)))) return st;
     if ((st = ds.pos.reserve(size_t(n_tets) * 12 * sizeof(float) + 4))) return st;
     if ((st = ds.stats.reserve(kNumStats * sizeof(unsigned long long)))) return st;
-    const GridGeom g = make_grid_geom(bbox6, dims);
+    const GridGeom g = make_grid_geom(bbox6, dims, cell_scaling());
     e = launch_fill_f64(ds.grid.as<double>(), int64_t(ncell), 0.0, ds.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(ds.pos.ptr, tets, size_t(n_tets) * 12 * sizeof(float), hipMemcpyHostToDevice, ds.stream);
     if (e == hipSuccess && stats) e = hipMemsetAsync(ds.stats.ptr, 0, kNumStats * sizeof(unsigned long long), ds.stream);

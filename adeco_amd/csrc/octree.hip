@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include <cstring>
 #include <list>
 #include <stack>
@@ -124,7 +125,7 @@ __global__ void k_refine_level(const float* __restrict__ mbox, const unsigned* _
 constexpr int kMaxStack = 8 * kMaxOctreeDepth + 8;
 
 template <bool kFill>
-__global__ void __launch_bounds__(256) k_bin_tets(LatticeView lat, CornerTable ct, int64_t n_tets,
+__global__ void __launch_bounds__(256) k_bin_tets(LatticeView lat, PackedCorners ct, int64_t n_tets,
                                                   const float* __restrict__ node_box, const int32_t* __restrict__ first_child,
                                                   const int32_t* __restrict__ leaf_index, int* __restrict__ leaf_count,
                                                   const int64_t* __restrict__ leaf_offset, int32_t* __restrict__ pairs) {
@@ -225,7 +226,15 @@ adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
     std::vector<float> boxes(6 * n_nodes);
     std::vector<int32_t> first(n_nodes), leaf_index(n_nodes, -1);
     for (int64_t i = 0; i < n_nodes; ++i) {
-        T->node_box(int(i), &boxes[6 * i]);
+        // widened by two float ulps per face: a conservative superset of the tets that strictly
+        // overlap either the float box or the exact dyadic box of the node.  Extra tets deposit
+        // nothing (the per-tet deposit re-tests strict overlap with its grid).
+        float* b = &boxes[6 * i];
+        T->node_box(int(i), b);
+        for (int a = 0; a < 3; ++a) {
+            b[2 * a] = std::nextafter(std::nextafter(b[2 * a], -FLT_MAX), -FLT_MAX);
+            b[2 * a + 1] = std::nextafter(std::nextafter(b[2 * a + 1], FLT_MAX), FLT_MAX);
+        }
         first[i] = T->nodes[i].first_child;
     }
     for (int64_t l = 0; l < n_leaves; ++l) leaf_index[T->leaves_dfs[l]] = int32_t(l);
@@ -241,7 +250,7 @@ adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
     if (e != hipSuccess) return hip_fail(e, "binning setup");
     const int64_t n_tets = 6 * int64_t(L->np[0] - 1) * (L->np[1] - 1) * (L->np[2] - 1);
     const LatticeView lv{L->d_x, L->d_y, L->d_z, {L->np[0], L->np[1], L->np[2]}};
-    const CornerTable ct = corner_table(decomposition);
+    const PackedCorners ct = pack_corners(corner_table(decomposition));
     const unsigned nb = unsigned((n_tets + 255) / 256);
     hipLaunchKernelGGL(k_bin_tets<false>, dim3(nb), dim3(256), 0, ds.stream, lv, ct, n_tets, T->d_node_box.as<float>(),
                        T->d_first.as<int32_t>(), T->d_leaf_index.as<int32_t>(), T->d_leaf_count.as<int>(),

@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "kernels.hpp"
 #include "lattice_impl.hpp"
 #include "runtime.hpp"
 
@@ -22,6 +23,7 @@ struct ONode {
     int parent;
     int first_child;           // -1 when unrefined
     unsigned long long count;  // num_tets_covering_node_estimate (== num_tets_on_proc_estimate[0] on one rank)
+    int ix[3];                 // integer position among the 2^level nodes per axis of its level
 };
 
 }  // namespace adeco
@@ -40,7 +42,9 @@ struct adeco_octree_s {
     adeco::DevBuf d_mc_box, d_mc_cells, d_node_box, d_first, d_leaf_index, d_leaf_count, d_leaf_offset, d_pairs;
     // leaf grids of the last deposit (device, FP64, leaf-major in DFS leaf order)
     int grid_dims = 0;
-    adeco::DevBuf d_grids, d_leaf_geom, d_work;
+    adeco::DevBuf d_grids, d_leaf_geom, d_work, d_tiles;
+    int64_t last_candidates = 0;            // (tet, cell) candidates of the last batched deposit
+    unsigned long long* d_fallback = nullptr;  // FP64 recomputes of the last mixed-precision deposit
 
     // (ll, ll + ext) per axis in float, as AdOctreeGeneration::getBBox (AdOctreeGeneration.h:67-73)
     void node_box(int id, float out[6]) const {
@@ -49,6 +53,34 @@ struct adeco_octree_s {
             out[2 * a] = n.ll[a];
             out[2 * a + 1] = n.ll[a] + n.ext;
         }
+    }
+
+    // The node's box as the exact dyadic subdivision of the root, in FP64: root_ll + ix * h,
+    // h = root_ext / 2^level.  Every such bound is exactly representable (a float times a power
+    // of two plus a float needs < 53 bits for level <= 20), so leaves share faces bit for bit and
+    // tile the root exactly, unlike the float boxes above whose neighbours at different levels can
+    // be an ulp apart.
+    void node_box_exact(int id, double out[6]) const {
+        const adeco::ONode& n = nodes[id];
+        const double h = double(nodes[0].ext) / double(1u << n.level);
+        for (int a = 0; a < 3; ++a) {
+            out[2 * a] = double(nodes[0].ll[a]) + double(n.ix[a]) * h;
+            out[2 * a + 1] = double(nodes[0].ll[a]) + double(n.ix[a] + 1) * h;
+        }
+    }
+
+    // Grid geometry of a leaf: float reference boxes with the reference's scaling, or the exact
+    // dyadic boxes with per-axis scaling.
+    adeco::GridGeom leaf_geom(int dfs_leaf_index, int dims, int scaling) const {
+        const int id = leaves_dfs[dfs_leaf_index];
+        if (scaling == adeco::kScalingReference) {
+            float b[6];
+            node_box(id, b);
+            return adeco::make_grid_geom(b, dims, scaling);
+        }
+        double b[6];
+        node_box_exact(id, b);
+        return adeco::make_grid_geom_d(b, dims, scaling);
     }
 };
 

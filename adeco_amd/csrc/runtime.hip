@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -23,8 +24,12 @@ namespace adeco {
 
 thread_local std::string g_last_error;
 thread_local double g_last_kernel_ms = 0.0;
+thread_local LastTimings g_timings;
 static size_t g_memory_budget = 0;  // 0 -> 75% of device memory
 static std::mutex g_budget_mutex;
+static std::atomic<int> g_cell_scaling{kScalingExact};
+
+int cell_scaling() { return g_cell_scaling.load(); }
 
 adeco_status fail(adeco_status st, const std::string& msg) {
     g_last_error = msg;
@@ -53,6 +58,8 @@ adeco_status DeviceState::ensure(int dev) {
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     e = hipEventCreate(&ev0);
     if (e == hipSuccess) e = hipEventCreate(&ev1);
+    if (e == hipSuccess) e = hipEventCreate(&evk0);
+    if (e == hipSuccess) e = hipEventCreate(&evk1);
     if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
     size_t free_b = 0, total_b = 0;
     e = hipMemGetInfo(&free_b, &total_b);
@@ -145,7 +152,7 @@ adeco_status deposit_patches_on_device(int dev, size_t first_patch, size_t n_pat
         return fail(ADECO_EOVERFLOW, "memory budget too small for one meta-patch");
     if ((st = ds.grid.reserve(grid_bytes))) return st;
     if ((st = ds.pos.reserve(patches_per_pass * patch_bytes))) return st;
-    const GridGeom g = make_grid_geom(bbox, dims);
+    const GridGeom g = make_grid_geom(bbox, dims, cell_scaling());
     if ((e = launch_fill_f64(ds.grid.as<double>(), int64_t(ncell), 0.0, ds.stream)) != hipSuccess)
         return hip_fail(e, "grid clear");
     double total_ms = 0.0;
@@ -193,6 +200,24 @@ adeco_status adeco_set_memory_budget(size_t bytes) {
 }
 
 double adeco_last_kernel_ms(void) { return g_last_kernel_ms; }
+
+adeco_status adeco_set_cell_scaling(int mode) {
+    if (mode != ADECO_SCALING_EXACT && mode != ADECO_SCALING_REFERENCE)
+        return fail(ADECO_EINVAL, "set_cell_scaling(): unknown mode");
+    g_cell_scaling.store(mode);
+    return ADECO_OK;
+}
+
+int adeco_get_cell_scaling(void) { return g_cell_scaling.load(); }
+
+void adeco_last_timings(double* out5) {
+    if (!out5) return;
+    out5[0] = g_timings.total_ms;
+    out5[1] = g_timings.main_ms;
+    out5[2] = g_timings.candidates;
+    out5[3] = g_timings.partials;
+    out5[4] = g_timings.fallbacks;
+}
 
 adeco_status adeco_deposit_patches(int use_specific_gpu, size_t device_id, size_t patch_offset,
                                    size_t num_patches, const float* positions, size_t positions_size,
@@ -256,7 +281,7 @@ double adeco_selftest_beyond_fraction_f64(const double* d) { return beyond_fract
 adeco_status adeco_selftest_deposit_soup_f64(const float* tets, int64_t n_tets, const float* bbox6, int dims,
                                              double tet_mass, double* grid) {
     if (!tets || !bbox6 || !grid || dims < 1 || !valid_bbox(bbox6)) return fail(ADECO_EINVAL, "selftest: invalid argument");
-    const GridGeom g = make_grid_geom(bbox6, dims);
+    const GridGeom g = make_grid_geom(bbox6, dims, cell_scaling());
     struct HostEmit {
         double* grid;
         void operator()(int64_t idx, double dm) { grid[idx] += dm; }

@@ -31,12 +31,20 @@ struct DevBuf {
     ~DevBuf();
 };
 
+// Timings of the calling thread's last deposit: whole deposit (events on the launch stream),
+// the dominant kernel alone, (tet, cell) candidates and FP64 recomputes of a mixed-precision run.
+struct LastTimings {
+    double total_ms = 0, main_ms = 0;
+    double candidates = 0, fallbacks = 0, partials = 0;
+};
+extern thread_local LastTimings g_timings;
+
 struct DeviceState {
     int id = -1;
     bool ready = false;
     std::mutex mutex;  // serialises calls that share this device's buffers and stream
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
     size_t total_mem = 0;
     DevBuf pos, grid, stats, scratch;
     adeco_status ensure(int dev);

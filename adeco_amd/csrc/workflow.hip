@@ -33,6 +33,7 @@ struct BNode {
     float ll[3];
     float ext;
     int level;
+    int ix[3];
     int bfs_parent;
     int child0 = -1;  // BFS index of child 0 when refined
     unsigned long long count = 0;
@@ -75,6 +76,7 @@ adeco_status build_tree(adeco_octree_s* T, const float* roi6, uint64_t max_tets_
     for (int a = 0; a < 3; ++a) root.ll[a] = gb[2 * a];
     root.ext = std::max(std::max(gb[1] - gb[0], gb[3] - gb[2]), gb[5] - gb[4]);
     root.level = 0;
+    root.ix[0] = root.ix[1] = root.ix[2] = 0;
     root.bfs_parent = -1;
     root.list.resize(M);
     for (int m = 0; m < M; ++m) root.list[m] = m;
@@ -112,6 +114,9 @@ adeco_status build_tree(adeco_octree_s* T, const float* roi6, uint64_t max_tets_
                         c.ll[2] = p.ll[2] + float(k) * child_ext;
                         c.ext = child_ext;
                         c.level = p.level + 1;
+                        c.ix[0] = 2 * p.ix[0] + i;
+                        c.ix[1] = 2 * p.ix[1] + j;
+                        c.ix[2] = 2 * p.ix[2] + k;
                         c.bfs_parent = refined[s];
                         box_of(c, &cboxes[6 * (8 * s + i + 2 * j + 4 * k)]);
                     }
@@ -159,6 +164,7 @@ adeco_status build_tree(adeco_octree_s* T, const float* roi6, uint64_t max_tets_
         o.parent = parent_dfs;
         o.first_child = -1;
         o.count = bn[b].count;
+        std::memcpy(o.ix, bn[b].ix, sizeof(o.ix));
         dfs_of[b] = int(T->nodes.size());
         T->nodes.push_back(o);
     };
@@ -217,12 +223,11 @@ adeco_status deposit_leaves_plain(adeco_octree_s* T, int dims, int decomposition
     const int64_t nl = int64_t(T->leaves_dfs.size());
     const int64_t cells = int64_t(dims) * dims * dims;
     const LatticeView lv{L->d_x, L->d_y, L->d_z, {L->np[0], L->np[1], L->np[2]}};
+    const int scaling = cell_scaling();
     for (int64_t l = 0; l < nl; ++l) {
         const int64_t b = T->leaf_tet_offset[l], n = T->leaf_tet_offset[l + 1] - b;
         if (!n) continue;
-        float box[6];
-        T->node_box(T->leaves_dfs[l], box);
-        const GridGeom g = make_grid_geom(box, dims);
+        const GridGeom g = T->leaf_geom(int(l), dims, scaling);
         hipError_t e = launch_deposit_lattice_plain(lv, decomposition, T->d_pairs.as<int32_t>() + b, n, g, tet_mass,
                                                     T->d_grids.as<double>() + l * cells, nullptr, ds.stream);
         if (e != hipSuccess) return hip_fail(e, "leaf deposit launch");
@@ -344,13 +349,58 @@ adeco_status adeco_deposit_leaves(adeco_octree T, int dims, int decomposition, d
     T->grid_dims = dims;
     e = launch_fill_f64(T->d_grids.as<double>(), int64_t(bytes / 8), 0.0, ds.stream);
     if (e != hipSuccess) return hip_fail(e, "leaf grid clear");
+    g_timings = LastTimings();
     if ((st = timed_begin(ds))) return st;
-    if (mode == ADECO_MODE_PLAIN && precision == ADECO_PREC_FP64)
+    const bool plain = mode == ADECO_MODE_PLAIN && precision == ADECO_PREC_FP64;
+    if (plain)
         st = deposit_leaves_plain(T, dims, decomposition, tet_mass);
     else
         st = deposit_leaves_batched(T, dims, decomposition, tet_mass, precision);
     if (st) return st;
-    return timed_end(ds);
+    if ((st = timed_end(ds))) return st;
+    g_timings.total_ms = g_last_kernel_ms;
+    g_timings.main_ms = g_last_kernel_ms;
+    if (!plain && T->d_fallback) {
+        float ms = 0.f;
+        unsigned long long cnt[2] = {0, 0};
+        e = hipEventElapsedTime(&ms, ds.evk0, ds.evk1);
+        if (e == hipSuccess) e = hipMemcpy(cnt, T->d_fallback, 16, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "batched timings");
+        g_timings.main_ms = ms;
+        g_timings.fallbacks = double(cnt[0]);
+        g_timings.partials = double(cnt[1]);
+        g_timings.candidates = double(T->last_candidates);
+    }
+    return ADECO_OK;
+}
+
+// Leaf boxes (DFS order, x0,x1,y0,y1,z0,z1 per leaf) as the deposit uses them: the float
+// reference boxes (exact = 0) or the exact dyadic FP64 boxes (exact = 1).
+adeco_status adeco_octree_leaf_boxes(adeco_octree T, int exact, double* out) {
+    if (!T || !out) return fail(ADECO_EINVAL, "octree_leaf_boxes(): null argument");
+    for (size_t l = 0; l < T->leaves_dfs.size(); ++l) {
+        if (exact) {
+            T->node_box_exact(T->leaves_dfs[l], out + 6 * l);
+        } else {
+            float b[6];
+            T->node_box(T->leaves_dfs[l], b);
+            for (int a = 0; a < 6; ++a) out[6 * l + a] = b[a];
+        }
+    }
+    return ADECO_OK;
+}
+
+// Binned lattice tet ids of one leaf (DFS index).  Returns the count; out may be null.
+int64_t adeco_octree_leaf_tets(adeco_octree T, int64_t leaf, int32_t* out) {
+    if (!T || T->binned_decomposition < 0 || leaf < 0 || leaf + 1 >= int64_t(T->leaf_tet_offset.size())) return -1;
+    const int64_t b = T->leaf_tet_offset[leaf], n = T->leaf_tet_offset[leaf + 1] - b;
+    if (out && n) {
+        std::lock_guard<std::mutex> lk(T->L->ds->mutex);
+        if (hipSetDevice(T->L->device) != hipSuccess ||
+            hipMemcpy(out, T->d_pairs.as<int32_t>() + b, size_t(n) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            return -1;
+    }
+    return n;
 }
 
 // Leaf grids of the last deposit, leaf-major in DFS leaf order, FP64 mass per cell.

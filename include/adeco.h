@@ -50,8 +50,17 @@ typedef int adeco_status;
 #define ADECO_PREC_MIXED 1 /* FP32 clip with FP64 recompute where the FP32 error bound is not met */
 #define ADECO_PREC_FP32 2  /* FP32 clip only (for the precision study) */
 
+/* Cell scaling of the clip (library-wide, read when a deposit starts).  The reference scales all
+ * three axes by 1/dx (AdDepositTets2Grid.h:569), assuming cubic cells; float boxes (ll, ll + ext)
+ * rarely have bit-identical extents, so that formula shifts cell faces by ~1e-7 of a cell and
+ * breaks mass conservation at that level.  EXACT (default) scales each axis by its own 1/delta. */
+#define ADECO_SCALING_EXACT 0
+#define ADECO_SCALING_REFERENCE 1
+
 const char* adeco_last_error(void);
 int adeco_version(void);
+adeco_status adeco_set_cell_scaling(int mode);
+int adeco_get_cell_scaling(void);
 /* Number of visible GPUs, or -1 when the HIP runtime reports an error. */
 int adeco_device_count(void);
 
@@ -133,6 +142,16 @@ adeco_status adeco_deposit_leaves(adeco_octree tree, int dims, int decomposition
 /* Copies leaf grids [first_leaf, first_leaf + n_leaves) of the last deposit (FP64 mass per cell,
  * i fastest, leaf-major in DFS leaf order). */
 adeco_status adeco_leaf_grids(adeco_octree tree, int64_t first_leaf, int64_t n_leaves, double* out);
+/* Leaf boxes in DFS order (6 doubles per leaf): exact = 0 gives the reference's float boxes
+ * (ll, ll + ext) that ADECO_SCALING_REFERENCE deposits into; exact = 1 the dyadic FP64
+ * subdivision of the root that ADECO_SCALING_EXACT uses, which tiles the root exactly. */
+adeco_status adeco_octree_leaf_boxes(adeco_octree tree, int exact, double* out);
+/* Binned lattice tet ids of one leaf (DFS index) after adeco_octree_bin / adeco_deposit_leaves;
+ * returns their number (-1 on error); out (nullable) receives them. */
+int64_t adeco_octree_leaf_tets(adeco_octree tree, int64_t leaf, int32_t* out);
+/* The calling thread's last leaf deposit: out5 = {total ms, dominant-kernel ms, (tet, cell)
+ * candidates, partial cells clipped, FP64 recomputes of the mixed-precision clip}. */
+void adeco_last_timings(double* out5);
 
 /* ---- self-test hooks: the shared host/device geometry evaluated on the host ----
  * These exist so the CPU test suite can check the clip and classification code that the kernels

@@ -32,6 +32,12 @@ void stats_out(const DepositStats& s, const OpCounts* ops, uint64_t* out) {
     if (ops) { out[12] += ops->flops(); out[13] += ops->add; out[14] += ops->mul; out[15] += ops->div; }
 }
 
+// FP64 cell scaling, numbered like the library's ADECO_SCALING_*: 0 = per axis (exact partition),
+// 1 = the reference's 1/dx on every axis (the default here).  The float-faithful mode always uses
+// the reference's formula.
+std::atomic<int> g_scaling{1};
+bool per_axis() { return g_scaling.load() == 0; }
+
 bool patches_ok(size_t patch_offset, size_t num_patches, size_t positions_size, int m) {
     if (m < 1) return false;
     const size_t fpp = 3 * size_t(m + 1) * (m + 1) * (m + 1);
@@ -42,6 +48,9 @@ bool patches_ok(size_t patch_offset, size_t num_patches, size_t positions_size, 
 }  // namespace
 
 extern "C" {
+
+void oracle_set_cell_scaling(int mode) { g_scaling.store(mode == 0 ? 0 : 1); }
+int oracle_get_cell_scaling() { return g_scaling.load(); }
 
 // Float-faithful restatement of DepositWrapper::deposit_tets (AdDeposit2Grid.cpp:67-125):
 // accumulates into a float grid in the reference's order.  Returns 0, or -1 on inconsistent
@@ -73,7 +82,7 @@ int oracle_deposit_patches_f64(size_t patch_offset, size_t num_patches, const fl
     for (size_t p = 0; p < num_patches; ++p)
         for (int c = 0; c < cpp; ++c)
             deposit_patch_cell<double, double>(c, positions + (patch_offset + p) * fpp, m, gb, dims,
-                                               tet_mass, grid, stats ? &st : nullptr);
+                                               tet_mass, grid, stats ? &st : nullptr, per_axis());
     stats_out(st, nullptr, stats);
     return 0;
 }
@@ -82,10 +91,26 @@ int oracle_deposit_patches_f64(size_t patch_offset, size_t num_patches, const fl
 // worker threads with private grids summed at the end (the reference's worker grids,
 // AdDynamicLoadBalancing.h:489-524).  precision: 0 = float-faithful into a float grid (gridf),
 // 1 = FP64 into a double grid (gridd), 2 = FP64 with operation counting (stats[12..15]).
+int oracle_deposit_lattice_d(const float* x, const float* y, const float* z, const int* np, int decomp,
+                             const int32_t* tet_ids, int64_t n_tets, const double* bbox, int dims,
+                             double tet_mass, int precision, float* gridf, double* gridd, int nthreads,
+                             uint64_t* stats);
+
 int oracle_deposit_lattice(const float* x, const float* y, const float* z, const int* np, int decomp,
                            const int32_t* tet_ids, int64_t n_tets, const float* bbox, int dims,
                            double tet_mass, int precision, float* gridf, double* gridd, int nthreads,
                            uint64_t* stats) {
+    const double b[6] = {bbox[0], bbox[1], bbox[2], bbox[3], bbox[4], bbox[5]};
+    return oracle_deposit_lattice_d(x, y, z, np, decomp, tet_ids, n_tets, b, dims, tet_mass, precision, gridf,
+                                    gridd, nthreads, stats);
+}
+
+// The same with an FP64 grid box (the float-faithful mode rounds it to float, exact for boxes that
+// came from floats).
+int oracle_deposit_lattice_d(const float* x, const float* y, const float* z, const int* np, int decomp,
+                             const int32_t* tet_ids, int64_t n_tets, const double* bbox, int dims,
+                             double tet_mass, int precision, float* gridf, double* gridd, int nthreads,
+                             uint64_t* stats) {
     int corners[6][4];
     decomposition_corners(decomp, corners);
     const int gd[3] = {dims, dims, dims};
@@ -112,13 +137,13 @@ int oracle_deposit_lattice(const float* x, const float* y, const float* z, const
                 if (precision == 0) {
                     float v[4][3];
                     lattice_tet(x, y, z, np, corners, t, v);
-                    deposit_tet<float, float>(v, float(tet_mass), gd, bbox6<float>(bbox), gf[tid].data(),
+                    deposit_tet<float, float>(v, float(tet_mass), gd, bbox6d<float>(bbox), gf[tid].data(),
                                               stats ? &sts[tid] : nullptr);
                 } else if (precision == 1) {
                     double v[4][3];
                     lattice_tet(x, y, z, np, corners, t, v);
-                    deposit_tet<double, double>(v, tet_mass, gd, bbox6<double>(bbox), gdv[tid].data(),
-                                                stats ? &sts[tid] : nullptr);
+                    deposit_tet<double, double>(v, tet_mass, gd, bbox6d<double>(bbox), gdv[tid].data(),
+                                                stats ? &sts[tid] : nullptr, per_axis());
                 } else {
                     using C = Counted<double>;
                     C v[4][3];
@@ -127,8 +152,8 @@ int oracle_deposit_lattice(const float* x, const float* y, const float* z, const
                     for (int i = 0; i < 4; ++i)
                         for (int a = 0; a < 3; ++a) v[i][a] = C(vd[i][a]);
                     Box<C> gb;
-                    for (int a = 0; a < 3; ++a) { gb.mn[a] = C(double(bbox[2 * a])); gb.mx[a] = C(double(bbox[2 * a + 1])); }
-                    deposit_tet<C, C>(v, C(tet_mass), gd, gb, gc[tid].data(), &sts[tid]);
+                    for (int a = 0; a < 3; ++a) { gb.mn[a] = C(bbox[2 * a]); gb.mx[a] = C(bbox[2 * a + 1]); }
+                    deposit_tet<C, C>(v, C(tet_mass), gd, gb, gc[tid].data(), &sts[tid], per_axis());
                 }
             }
         }
@@ -159,7 +184,7 @@ int oracle_deposit_soup_f64(const float* tets, int64_t n_tets, const float* bbox
         double v[4][3];
         for (int i = 0; i < 4; ++i)
             for (int a = 0; a < 3; ++a) v[i][a] = double(tets[12 * t + 3 * i + a]);
-        deposit_tet<double, double>(v, tet_mass, gd, gb, grid, stats ? &st : nullptr);
+        deposit_tet<double, double>(v, tet_mass, gd, gb, grid, stats ? &st : nullptr, per_axis());
     }
     stats_out(st, nullptr, stats);
     return 0;

@@ -312,10 +312,12 @@ struct DepositStats {
 template <class T>
 inline int clampi(int v, int lo, int hi) { return v <= lo ? lo : (v >= hi ? hi : v); }
 
-// AdDepositTets2Grid.h:509-665 (grid index i + dims*(j + dims*k), accumulate with +=)
+// AdDepositTets2Grid.h:509-665 (grid index i + dims*(j + dims*k), accumulate with +=).
+// per_axis_scale = false is the reference's scale = 1/dx on every axis (AdDepositTets2Grid.h:569);
+// true scales each axis by its own 1/delta (an exact partition of a non-cubic float box).
 template <class G, class T>
 inline void deposit_tet(const T v[4][3], G tet_mass, const int dims[3], const Box<T>& gb, G* grid,
-                        DepositStats* st) {
+                        DepositStats* st, bool per_axis_scale = false) {
     if (st) ++st->tets;
     Box<T> tb;
     for (int a = 0; a < 3; ++a) tb.mn[a] = tb.mx[a] = v[0][a];
@@ -346,7 +348,8 @@ inline void deposit_tet(const T v[4][3], G tet_mass, const int dims[3], const Bo
     for (int i = 0; i < 4; ++i)
         for (int a = 0; a < 3; ++a) src[i][a] = v[ord[i]][a];
     if (st) ++st->tets_active;
-    const T scale = T(1.f) / delta[0];
+    T scale[3];
+    for (int a = 0; a < 3; ++a) scale[a] = T(1.f) / delta[per_axis_scale ? a : 0];
     const T abs_vol = fabs_t(tet_vol);
     Box<T> cb;
     for (int k = cov[2][0]; k <= cov[2][1]; ++k) {
@@ -363,7 +366,7 @@ inline void deposit_tet(const T v[4][3], G tet_mass, const int dims[3], const Bo
                 if (st) ++st->aabb_pass;
                 T s[4][3];
                 for (int q = 0; q < 4; ++q)
-                    for (int a = 0; a < 3; ++a) s[q][a] = scale * (src[q][a] - cb.mn[a]);
+                    for (int a = 0; a < 3; ++a) s[q][a] = scale[a] * (src[q][a] - cb.mn[a]);
                 const Intersection it = intersection_test(s);
                 G dmass = G(0.f);
                 if (it == NO_INTERSECTION) {
@@ -396,7 +399,7 @@ inline void deposit_tet(const T v[4][3], G tet_mass, const int dims[3], const Bo
 // AdDepositTets2Grid.h:917-1053 and AdDeposit2Grid.cpp:67-125
 template <class G, class T>
 inline void deposit_patch_cell(int cell, const float* patch, int m, const Box<T>& gb, int dims,
-                               G tet_mass, G* grid, DepositStats* st) {
+                               G tet_mass, G* grid, DepositStats* st, bool per_axis_scale = false) {
     const int mm = m * m;
     const int k = cell / mm, rest = cell % mm, j = rest / m, i = rest % m;
     const int base = 3 * (i + (m + 1) * (j + k * (m + 1)));
@@ -413,12 +416,21 @@ inline void deposit_patch_cell(int cell, const float* patch, int m, const Box<T>
         T v[4][3];
         for (int q = 0; q < 4; ++q)
             for (int a = 0; a < 3; ++a) v[q][a] = T(cp[kTetConn[t][q]][a]);
-        deposit_tet<G, T>(v, tet_mass, gd, gb, grid, st);
+        deposit_tet<G, T>(v, tet_mass, gd, gb, grid, st, per_axis_scale);
     }
 }
 
 template <class T>
 inline Box<T> bbox6(const float* b) {
+    Box<T> g;
+    g.mn[0] = T(b[0]); g.mx[0] = T(b[1]);
+    g.mn[1] = T(b[2]); g.mx[1] = T(b[3]);
+    g.mn[2] = T(b[4]); g.mx[2] = T(b[5]);
+    return g;
+}
+
+template <class T>
+inline Box<T> bbox6d(const double* b) {
     Box<T> g;
     g.mn[0] = T(b[0]); g.mx[0] = T(b[1]);
     g.mn[1] = T(b[2]); g.mx[1] = T(b[3]);

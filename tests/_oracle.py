@@ -23,6 +23,7 @@ _ORACLE_SIGS = {
     "oracle_deposit_patches_f32": (cint, [sz, sz, vp, sz, cint, vp, cint, vp]),
     "oracle_deposit_patches_f64": (cint, [sz, sz, vp, sz, cint, vp, cint, dbl, vp, vp]),
     "oracle_deposit_lattice": (cint, [vp, vp, vp, vp, cint, vp, i64, vp, cint, dbl, cint, vp, vp, cint, vp]),
+    "oracle_deposit_lattice_d": (cint, [vp, vp, vp, vp, cint, vp, i64, vp, cint, dbl, cint, vp, vp, cint, vp]),
     "oracle_deposit_soup_f64": (cint, [vp, i64, vp, cint, dbl, vp, vp]),
     "oracle_clip_unit_cube_f32": (ctypes.c_float, [vp]),
     "oracle_clip_unit_cube_f64": (dbl, [vp, vp]),
@@ -39,7 +40,36 @@ _ORACLE_SIGS = {
     "oracle_octree_leaves": (i64, [vp, vp, vp, vp, vp]),
     "oracle_meta_cell_bboxes": (None, [vp, vp]),
     "oracle_estimate": (u64, [vp, vp, i64]),
+    "oracle_set_cell_scaling": (None, [cint]),
+    "oracle_get_cell_scaling": (cint, []),
 }
+
+EXACT = 0  # per-axis cell scaling (library default)
+REFERENCE = 1  # the reference's 1/dx on every axis (oracle default)
+
+
+class cell_scaling:
+    """Context manager: sets the oracle's FP64 cell scaling (and the library's, when loaded)."""
+
+    def __init__(self, mode, library=True):
+        self.mode = mode
+        self.library = library
+
+    def __enter__(self):
+        o = load()
+        self.prev = o.oracle_get_cell_scaling()
+        o.oracle_set_cell_scaling(self.mode)
+        if self.library:
+            import adeco_amd as A
+            self.prev_lib = A.get_cell_scaling()
+            A.set_cell_scaling(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        load().oracle_set_cell_scaling(self.prev)
+        if self.library:
+            import adeco_amd as A
+            A.set_cell_scaling(self.prev_lib)
 
 _REF_SIGS = {
     "ref_last_error": (ctypes.c_char_p, []),
@@ -110,17 +140,19 @@ def deposit_patches_f32(patches, bbox, dims, patch_offset=0, num_patches=None):
 
 
 def deposit_lattice(lat, bbox, dims, decomp=0, tet_mass=1.0, tet_ids=None, precision=1, nthreads=1, stats=False):
-    """precision 0 = float-faithful (float grid), 1 = FP64, 2 = FP64 with op counting."""
+    """precision 0 = float-faithful (float grid), 1 = FP64, 2 = FP64 with op counting.
+    A float64 bbox is passed to the oracle as FP64 (exact dyadic leaf boxes); otherwise float32."""
     o = load()
     np_ = np.array(lat.dims, np.int32)
-    bbox = np.ascontiguousarray(bbox, np.float32)
+    fn = o.oracle_deposit_lattice_d if np.asarray(bbox).dtype == np.float64 else o.oracle_deposit_lattice
+    bbox = np.ascontiguousarray(bbox, np.float64 if fn is o.oracle_deposit_lattice_d else np.float32)
     ids = None if tet_ids is None else np.ascontiguousarray(tet_ids, np.int32)
     n = lat.num_tets if ids is None else ids.size
     gf = np.zeros(dims ** 3, np.float32) if precision == 0 else None
     gd = np.zeros(dims ** 3, np.float64) if precision != 0 else None
     st = np.zeros(16, np.uint64)
-    o.oracle_deposit_lattice(P(lat.x), P(lat.y), P(lat.z), P(np_), decomp, P(ids), n, P(bbox), dims, tet_mass,
-                             precision, P(gf), P(gd), nthreads, P(st))
+    fn(P(lat.x), P(lat.y), P(lat.z), P(np_), decomp, P(ids), n, P(bbox), dims, tet_mass, precision, P(gf), P(gd),
+       nthreads, P(st))
     g = gf if precision == 0 else gd
     return (g, st) if stats else g
 

@@ -1,0 +1,83 @@
+"""GPU leaf deposit (deposit_leaf_nodes on one rank) against the FP64 oracle.
+
+For a refined octree, every leaf grid of the plain per-leaf launches, of the batched persistent
+launch in FP64 and of the mixed-precision form is compared with the oracle's FP64 deposit of the
+same leaf, per cell: |gpu - oracle| <= 1e-6 * max(|oracle|, 1e-8 * tet_mass).
+
+Exact geometry (the default): leaves are the dyadic FP64 subdivision of the root and cells are
+scaled per axis, so the leaves partition the root exactly and the total over all leaves equals the
+oracle's mass of the lattice inside the root box to 1e-9.  Reference geometry: the reference's
+float leaf boxes and 1/dx scaling, compared with the oracle in the same mode.
+"""
+
+import numpy as np
+import pytest
+
+import adeco_amd as A
+from adeco_amd.lattice import make_lattice
+from tests import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-6
+TET_MASS = 1.0 / 6.0
+
+
+def oracle_leaves(lat, tree, decomp, dims, leaves, exact=True):
+    boxes = tree.leaf_boxes_f64(exact=exact)
+    out = []
+    for l in leaves:
+        ids = tree.leaf_tets(int(l))
+        box = boxes[l] if exact else boxes[l].astype(np.float32)
+        out.append(O.deposit_lattice(lat, box, dims, decomp=decomp, tet_mass=TET_MASS, tet_ids=ids))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("decomp", [A.DECOMP_KUHN, A.DECOMP_REF_EFFECTIVE])
+def test_leaf_deposit_forms_match_oracle(decomp):
+    lat = make_lattice(24, 3.0, seed=4)
+    dims = 16
+    print(f"[leaves] decomp={decomp}", flush=True)
+    with O.cell_scaling(O.EXACT), A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 1500, 4) as tree:
+        nl = tree.num_leaves
+        assert nl > 8
+        counts = tree.bin(decomp)
+        pick = np.unique(np.concatenate([np.argsort(-counts)[:4], np.linspace(0, nl - 1, 6).astype(int)]))
+        want = oracle_leaves(lat, tree, decomp, dims, pick)
+        inside = O.deposit_lattice(lat, tree.root_box_f64(), 1, decomp=decomp, tet_mass=TET_MASS).sum()
+        for name, mode, prec in (("plain", A.MODE_PLAIN, A.PREC_FP64), ("batched", A.MODE_BATCHED, A.PREC_FP64),
+                                 ("mixed", A.MODE_BATCHED, A.PREC_MIXED)):
+            tree.deposit(dims, decomposition=decomp, tet_mass=TET_MASS, mode=mode, precision=prec)
+            g = tree.leaf_grids()
+            err = O.max_rel_err(g[pick], want, tet_mass=TET_MASS)
+            drift = abs(g.sum() - inside) / inside
+            print(f"  {name}: max rel err {err:.2e}, mass drift {drift:.2e}", flush=True)
+            assert err <= REL, name
+            assert drift <= 1e-9, name
+        tl = A.Octree.last_timings()
+        assert tl["partials"] > 0 and 0 <= tl["fallbacks"] <= tl["partials"]
+
+
+def test_reference_geometry_mode_matches_oracle():
+    lat = make_lattice(16, 2.0, seed=9)
+    dims = 8
+    with O.cell_scaling(O.REFERENCE), A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 2000, 3) as tree:
+        tree.bin(A.DECOMP_KUHN)
+        pick = np.arange(min(tree.num_leaves, 12))
+        want = oracle_leaves(lat, tree, A.DECOMP_KUHN, dims, pick, exact=False)
+        for mode in (A.MODE_PLAIN, A.MODE_BATCHED):
+            tree.deposit(dims, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=mode)
+            assert O.max_rel_err(tree.leaf_grids()[pick], want, tet_mass=TET_MASS) <= REL
+
+
+def test_batched_is_repeatable_and_dims_change():
+    lat = make_lattice(20, 2.5, seed=12)
+    with A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 1200, 4) as tree:
+        tree.deposit(8, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=A.MODE_BATCHED)
+        a = tree.leaf_grids()
+        tree.deposit(12, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=A.MODE_BATCHED)
+        b = tree.leaf_grids()
+        tree.deposit(8, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=A.MODE_BATCHED)
+        c = tree.leaf_grids()
+    assert O.max_rel_err(c, a, tet_mass=TET_MASS) <= 1e-12
+    assert abs(a.sum() - b.sum()) <= 1e-9 * a.sum()

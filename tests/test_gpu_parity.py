@@ -18,6 +18,28 @@ REL = 1e-6
 MASS_REL = 1e-9
 
 
+@pytest.fixture(autouse=True)
+def exact_scaling():
+    """Library and oracle both in per-axis cell scaling (the library default)."""
+    with O.cell_scaling(O.EXACT):
+        yield
+
+
+def test_boundary_reference_scaling_mode():
+    lat = make_lattice(10, 2.0, seed=3)
+    pat = meta_patches(lat)
+    bbox = A.cubic_root_box(lat.bbox())
+    with O.cell_scaling(O.REFERENCE):
+        data = np.zeros(16 ** 3, np.float32)
+        A.DepositWrapper.deposit_tets(True, 0, 0, pat.size // 1536, pat, pat.size, 7, bbox, 16, data)
+        want = O.deposit_patches_f64(pat, bbox, 16)
+    assert O.max_rel_err(data, want) <= REL
+    # the two scalings differ only through the box's float extents (one ulp of ~100 Mpc/h shifts
+    # a cell face by ~1e-7 of a cell; low-mass cells next to such a face see up to ~1e-5)
+    exact = O.deposit_patches_f64(pat, bbox, 16)
+    assert 0 < O.max_rel_err(exact, want) <= 1e-4
+
+
 def padded_box(bbox):
     """A cube strictly containing bbox with integer corners and a power-of-two edge, so its three
     cell sizes are identical in float (mass conservation then holds to rounding)."""
