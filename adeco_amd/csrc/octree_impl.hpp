@@ -42,7 +42,7 @@ struct adeco_octree_s {
     adeco::DevBuf d_mc_box, d_mc_cells, d_node_box, d_first, d_leaf_index, d_leaf_count, d_leaf_offset, d_pairs;
     // leaf grids of the last deposit (device, FP64, leaf-major in DFS leaf order)
     int grid_dims = 0;
-    adeco::DevBuf d_grids, d_leaf_geom, d_work, d_tiles;
+    adeco::DevBuf d_grids, d_leaf_geom, d_work, d_tiles, d_counters;
     int64_t last_candidates = 0;            // (tet, cell) candidates of the last batched deposit
     unsigned long long* d_fallback = nullptr;  // FP64 recomputes of the last mixed-precision deposit
 

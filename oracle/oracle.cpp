@@ -87,10 +87,29 @@ int oracle_deposit_patches_f64(size_t patch_offset, size_t num_patches, const fl
     return 0;
 }
 
+// The same in x87 extended precision, rounded to double when added into `grid`.
+int oracle_deposit_patches_ld(size_t patch_offset, size_t num_patches, const float* positions,
+                              size_t positions_size, int m, const float* bbox, int dims,
+                              double tet_mass, double* grid) {
+    if (positions_size == 0) return 0;
+    if (!patches_ok(patch_offset, num_patches, positions_size, m)) return -1;
+    const size_t fpp = 3 * size_t(m + 1) * (m + 1) * (m + 1);
+    const int cpp = m * m * m;
+    const Box<long double> gb = bbox6<long double>(bbox);
+    std::vector<long double> g(size_t(dims) * dims * dims, 0.L);
+    for (size_t p = 0; p < num_patches; ++p)
+        for (int c = 0; c < cpp; ++c)
+            deposit_patch_cell<long double, long double>(c, positions + (patch_offset + p) * fpp, m, gb, dims,
+                                                         tet_mass, g.data(), nullptr, per_axis());
+    for (size_t c = 0; c < g.size(); ++c) grid[c] += double(g[c]);
+    return 0;
+}
+
 // Deposit of lattice tets (or of the listed tet ids) onto one uniform grid, split over nthreads
 // worker threads with private grids summed at the end (the reference's worker grids,
 // AdDynamicLoadBalancing.h:489-524).  precision: 0 = float-faithful into a float grid (gridf),
-// 1 = FP64 into a double grid (gridd), 2 = FP64 with operation counting (stats[12..15]).
+// 1 = FP64 into a double grid (gridd), 2 = FP64 with operation counting (stats[12..15]),
+// 3 = x87 extended precision, rounded to double into gridd.
 int oracle_deposit_lattice_d(const float* x, const float* y, const float* z, const int* np, int decomp,
                              const int32_t* tet_ids, int64_t n_tets, const double* bbox, int dims,
                              double tet_mass, int precision, float* gridf, double* gridd, int nthreads,
@@ -119,6 +138,7 @@ int oracle_deposit_lattice_d(const float* x, const float* y, const float* z, con
     std::vector<std::vector<float>> gf(nthreads);
     std::vector<std::vector<double>> gdv(nthreads);
     std::vector<std::vector<Counted<double>>> gc(nthreads);
+    std::vector<std::vector<long double>> gl(nthreads);
     std::vector<DepositStats> sts(nthreads);
     std::vector<OpCounts> ops(nthreads);
     std::atomic<int64_t> next(0);
@@ -126,6 +146,7 @@ int oracle_deposit_lattice_d(const float* x, const float* y, const float* z, con
     auto work = [&](int tid) {
         if (precision == 0) gf[tid].assign(ncell, 0.f);
         else if (precision == 1) gdv[tid].assign(ncell, 0.);
+        else if (precision == 3) gl[tid].assign(ncell, 0.L);
         else gc[tid].assign(ncell, Counted<double>(0.));
         if (precision == 2) g_ops = &ops[tid];
         for (;;) {
@@ -144,6 +165,11 @@ int oracle_deposit_lattice_d(const float* x, const float* y, const float* z, con
                     lattice_tet(x, y, z, np, corners, t, v);
                     deposit_tet<double, double>(v, tet_mass, gd, bbox6d<double>(bbox), gdv[tid].data(),
                                                 stats ? &sts[tid] : nullptr, per_axis());
+                } else if (precision == 3) {
+                    long double v[4][3];
+                    lattice_tet(x, y, z, np, corners, t, v);
+                    deposit_tet<long double, long double>(v, tet_mass, gd, bbox6d<long double>(bbox), gl[tid].data(),
+                                                          stats ? &sts[tid] : nullptr, per_axis());
                 } else {
                     using C = Counted<double>;
                     C v[4][3];
@@ -168,9 +194,27 @@ int oracle_deposit_lattice_d(const float* x, const float* y, const float* z, con
     for (int t = 0; t < nthreads; ++t) {
         if (precision == 0) { for (size_t c = 0; c < ncell; ++c) gridf[c] += gf[t][c]; }
         else if (precision == 1) { for (size_t c = 0; c < ncell; ++c) gridd[c] += gdv[t][c]; }
+        else if (precision == 3) { for (size_t c = 0; c < ncell; ++c) gridd[c] += double(gl[t][c]); }
         else { for (size_t c = 0; c < ncell; ++c) gridd[c] += gc[t][c].v; }
         stats_out(sts[t], precision == 2 ? &ops[t] : nullptr, stats);
     }
+    return 0;
+}
+
+// Deposit of a tet soup in x87 extended precision (64-bit significand), rounded to double at the
+// end: the reference algorithm with its FP64 cancellation in slivers pushed ~2000x lower.
+int oracle_deposit_soup_ld(const float* tets, int64_t n_tets, const float* bbox, int dims, double tet_mass,
+                           double* grid) {
+    const int gd[3] = {dims, dims, dims};
+    const Box<long double> gb = bbox6<long double>(bbox);
+    std::vector<long double> g(size_t(dims) * dims * dims, 0.L);
+    for (int64_t t = 0; t < n_tets; ++t) {
+        long double v[4][3];
+        for (int i = 0; i < 4; ++i)
+            for (int a = 0; a < 3; ++a) v[i][a] = (long double)tets[12 * t + 3 * i + a];
+        deposit_tet<long double, long double>(v, tet_mass, gd, gb, g.data(), nullptr, per_axis());
+    }
+    for (size_t c = 0; c < g.size(); ++c) grid[c] += double(g[c]);
     return 0;
 }
 
@@ -190,7 +234,7 @@ int oracle_deposit_soup_f64(const float* tets, int64_t n_tets, const float* bbox
     return 0;
 }
 
-// Unit-cube clip of one tet (12 coordinates) in float-faithful or FP64 arithmetic.
+// Unit-cube clip of one tet (12 coordinates) in float-faithful, FP64 or extended arithmetic.
 float oracle_clip_unit_cube_f32(const float* t) {
     float v[4][3];
     std::memcpy(v, t, sizeof(v));
@@ -207,6 +251,12 @@ double oracle_clip_unit_cube_f64(const double* t, uint64_t* faces_edges_vols) {
         faces_edges_vols[2] = st.subtet_vols;
     }
     return r;
+}
+double oracle_clip_unit_cube_ld(const double* t) {
+    long double v[4][3];
+    for (int i = 0; i < 4; ++i)
+        for (int a = 0; a < 3; ++a) v[i][a] = t[3 * i + a];
+    return double(clip_unit_cube<long double>(v));
 }
 int oracle_intersection_test_f64(const double* t) {
     double v[4][3];

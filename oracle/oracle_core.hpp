@@ -64,6 +64,7 @@ struct Counted {
 };
 
 template <class T> struct NumTraits { using D = double; static double dbl(T x) { return double(x); } };
+template <> struct NumTraits<long double> { using D = long double; static long double dbl(long double x) { return x; } };
 template <class R> struct NumTraits<Counted<R>> {
     using D = Counted<R>;
     static Counted<R> dbl(Counted<R> x) { return x; }
@@ -71,6 +72,9 @@ template <class R> struct NumTraits<Counted<R>> {
 
 inline float fabs_t(float x) { return std::fabs(x); }
 inline double fabs_t(double x) { return std::fabs(x); }
+inline long double fabs_t(long double x) { return std::fabs(x); }
+inline long double floor_t(long double x) { return std::floor(x); }
+inline long double ceil_t(long double x) { return std::ceil(x); }
 template <class R> Counted<R> fabs_t(Counted<R> x) { return Counted<R>(std::fabs(x.v)); }
 inline float floor_t(float x) { return std::floor(x); }
 inline double floor_t(double x) { return std::floor(x); }

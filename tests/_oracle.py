@@ -25,8 +25,11 @@ _ORACLE_SIGS = {
     "oracle_deposit_lattice": (cint, [vp, vp, vp, vp, cint, vp, i64, vp, cint, dbl, cint, vp, vp, cint, vp]),
     "oracle_deposit_lattice_d": (cint, [vp, vp, vp, vp, cint, vp, i64, vp, cint, dbl, cint, vp, vp, cint, vp]),
     "oracle_deposit_soup_f64": (cint, [vp, i64, vp, cint, dbl, vp, vp]),
+    "oracle_deposit_soup_ld": (cint, [vp, i64, vp, cint, dbl, vp]),
+    "oracle_deposit_patches_ld": (cint, [sz, sz, vp, sz, cint, vp, cint, dbl, vp]),
     "oracle_clip_unit_cube_f32": (ctypes.c_float, [vp]),
     "oracle_clip_unit_cube_f64": (dbl, [vp, vp]),
+    "oracle_clip_unit_cube_ld": (dbl, [vp]),
     "oracle_intersection_test_f64": (cint, [vp]),
     "oracle_vol6_f64": (dbl, [vp]),
     "oracle_decomposition": (None, [cint, vp]),
@@ -129,6 +132,18 @@ def deposit_patches_f64(patches, bbox, dims, tet_mass=1.0, patch_offset=0, num_p
     return (g, st) if stats else g
 
 
+def deposit_patches_ld(patches, bbox, dims, tet_mass=1.0, patch_offset=0, num_patches=None):
+    """deposit_patches_f64 in x87 extended precision (64-bit significand), rounded to double."""
+    o = load()
+    patches = np.ascontiguousarray(patches, np.float32)
+    bbox = np.ascontiguousarray(bbox, np.float32)
+    n = patches.size // 1536 if num_patches is None else num_patches
+    g = np.zeros(dims ** 3, np.float64)
+    assert o.oracle_deposit_patches_ld(patch_offset, n, P(patches), patches.size, 7, P(bbox), dims, tet_mass,
+                                       P(g)) == 0
+    return g
+
+
 def deposit_patches_f32(patches, bbox, dims, patch_offset=0, num_patches=None):
     o = load()
     patches = np.ascontiguousarray(patches, np.float32)
@@ -140,7 +155,8 @@ def deposit_patches_f32(patches, bbox, dims, patch_offset=0, num_patches=None):
 
 
 def deposit_lattice(lat, bbox, dims, decomp=0, tet_mass=1.0, tet_ids=None, precision=1, nthreads=1, stats=False):
-    """precision 0 = float-faithful (float grid), 1 = FP64, 2 = FP64 with op counting.
+    """precision 0 = float-faithful (float grid), 1 = FP64, 2 = FP64 with op counting,
+    3 = x87 extended precision rounded to double.
     A float64 bbox is passed to the oracle as FP64 (exact dyadic leaf boxes); otherwise float32."""
     o = load()
     np_ = np.array(lat.dims, np.int32)
@@ -163,6 +179,34 @@ def deposit_soup_f64(tets, bbox, dims, tet_mass=1.0):
     bbox = np.ascontiguousarray(bbox, np.float32)
     g = np.zeros(dims ** 3, np.float64)
     o.oracle_deposit_soup_f64(P(tets), tets.shape[0], P(bbox), dims, tet_mass, P(g), None)
+    return g
+
+
+def lattice_tets(lat, decomp):
+    """All tets of a lattice as a float32 soup [6 n^3, 4, 3], tet id = 6 * cube + local tet, cube
+    index i fastest (the oracle's and the library's numbering)."""
+    c = np.zeros(24, np.int32)
+    load().oracle_decomposition(decomp, P(c))
+    n = lat.n
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    xyz = np.stack([lat.x, lat.y, lat.z], 1)
+    out = np.empty((n ** 3, 6, 4, 3), np.float32)
+    for t in range(6):
+        for q in range(4):
+            code = int(c[4 * t + q])
+            vi = (i + (code & 1)) + (n + 1) * ((j + ((code >> 1) & 1)) + (n + 1) * (k + ((code >> 2) & 1)))
+            out[:, t, q] = xyz[vi]
+    return out.reshape(-1, 4, 3)
+
+
+def deposit_soup_ld(tets, bbox, dims, tet_mass=1.0):
+    """The reference algorithm in x87 extended precision (64-bit significand), rounded to double."""
+    o = load()
+    tets = np.ascontiguousarray(tets, np.float32).reshape(-1, 12)
+    bbox = np.ascontiguousarray(bbox, np.float32)
+    g = np.zeros(dims ** 3, np.float64)
+    assert o.oracle_deposit_soup_ld(P(tets), tets.shape[0], P(bbox), dims, tet_mass, P(g)) == 0
     return g
 
 

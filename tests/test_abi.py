@@ -149,27 +149,18 @@ def test_kernel_classification_agrees_with_clip():
 
 @pytest.mark.parametrize("rms,dims", [(1.0, 8), (3.0, 20)])
 def test_kernel_per_tet_deposit_matches_oracle_on_host(rms, dims):
-    """The kernels' per-tet FP64 deposit routine, run on the host, against the FP64 oracle."""
+    """The kernels' per-tet FP64 deposit routine, run on the host, against the oracle."""
     lat = make_lattice(6, rms, seed=8)
-    o = O.load()
-    c = np.zeros(24, np.int32)
-    o.oracle_decomposition(2, O.P(c))
-    c = c.reshape(6, 4)
-    n, npv = lat.n, lat.n + 1
-    X = np.stack([lat.x, lat.y, lat.z], 1)
-    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
-    base = (i + npv * (j + npv * k)).ravel()
-    tets = np.stack([np.stack([X[base + (q & 1) + npv * (((q >> 1) & 1) + npv * ((q >> 2) & 1))] for q in c[t]], 1)
-                     for t in range(6)], 1).reshape(-1, 4, 3)
-    tets = np.ascontiguousarray(tets, np.float32)
+    tets = O.lattice_tets(lat, 2)
     box = np.array([-64, 192, -64, 192, -64, 192], np.float32)  # holds every tet; equal float cell sizes
     got = np.zeros(dims ** 3)
     assert A.load().adeco_selftest_deposit_soup_f64(A.ptr(tets), tets.shape[0], A.ptr(box), dims, 1.0, A.ptr(got)) == 0
-    want = O.deposit_soup_f64(tets, box, dims)
-    # same arithmetic class on both sides: cells above 1e-6 of a tet mass agree to 1e-9, and the
+    want = O.deposit_soup_ld(tets, box, dims)
+    # against the extended-precision oracle: cells above 1e-6 of a tet mass agree to 1e-9, and the
     # standard 1e-6 bar holds with the 1e-8 floor
     assert O.max_rel_err(got, want, floor_frac=1e-6) <= 1e-9
     assert O.max_rel_err(got, want) <= 1e-6
-    assert got.sum() == pytest.approx(6 * n ** 3, rel=1e-12)
+    assert O.max_rel_err(got, O.deposit_soup_f64(tets, box, dims), floor_frac=1e-6) <= 1e-9
+    assert got.sum() == pytest.approx(6 * lat.n ** 3, rel=1e-12)
     # the soup in lattice order equals the oracle's lattice deposit
     assert O.max_rel_err(got, O.deposit_lattice(lat, box, dims, decomp=2), floor_frac=1e-6) <= 1e-9

@@ -1,8 +1,9 @@
-"""GPU leaf deposit (deposit_leaf_nodes on one rank) against the FP64 oracle.
+"""GPU leaf deposit (deposit_leaf_nodes on one rank) against the oracle (the reference algorithm
+evaluated with a 64-bit significand, rounded to double).
 
 For a refined octree, every leaf grid of the plain per-leaf launches, of the batched persistent
-launch in FP64 and of the mixed-precision form is compared with the oracle's FP64 deposit of the
-same leaf, per cell: |gpu - oracle| <= 1e-6 * max(|oracle|, 1e-8 * tet_mass).
+launch in FP64 and of the mixed-precision form is compared with the oracle's deposit of the same
+leaf, per cell: |gpu - oracle| <= 1e-6 * max(|oracle|, 1e-8 * tet_mass).
 
 Exact geometry (the default): leaves are the dyadic FP64 subdivision of the root and cells are
 scaled per axis, so the leaves partition the root exactly and the total over all leaves equals the
@@ -29,7 +30,7 @@ def oracle_leaves(lat, tree, decomp, dims, leaves, exact=True):
     for l in leaves:
         ids = tree.leaf_tets(int(l))
         box = boxes[l] if exact else boxes[l].astype(np.float32)
-        out.append(O.deposit_lattice(lat, box, dims, decomp=decomp, tet_mass=TET_MASS, tet_ids=ids))
+        out.append(O.deposit_lattice(lat, box, dims, decomp=decomp, tet_mass=TET_MASS, tet_ids=ids, precision=3))
     return np.array(out)
 
 

@@ -1,8 +1,12 @@
-"""GPU parity of the deposit kernels against the FP64 oracle (tests/_oracle.py).
+"""GPU parity of the deposit kernels against the oracle.
 
-Tolerances (stated per test): per-cell relative error <= 1e-6 against the FP64 oracle, computed
-as |gpu - oracle| / max(|oracle|, 1e-8 * tet_mass); total mass conserved to 1e-9 relative when
-every tet lies inside the grid.
+The oracle here is the reference's algorithm evaluated with a 64-bit significand (x87 extended
+precision, rounded to double at the end): the FP64 evaluation of the same algorithm loses ~1e-13 of
+a tet mass to cancellation in sliver cells, which is above the bar in cells holding ~1e-8 of a tet.
+
+Tolerances (stated per test): per-cell relative error <= 1e-6, computed as
+|gpu - oracle| / max(|oracle|, 1e-8 * tet_mass); total mass conserved to 1e-9 relative when every
+tet lies inside the grid.
 """
 
 import numpy as np
@@ -32,11 +36,11 @@ def test_boundary_reference_scaling_mode():
     with O.cell_scaling(O.REFERENCE):
         data = np.zeros(16 ** 3, np.float32)
         A.DepositWrapper.deposit_tets(True, 0, 0, pat.size // 1536, pat, pat.size, 7, bbox, 16, data)
-        want = O.deposit_patches_f64(pat, bbox, 16)
+        want = O.deposit_patches_ld(pat, bbox, 16)
     assert O.max_rel_err(data, want) <= REL
     # the two scalings differ only through the box's float extents (one ulp of ~100 Mpc/h shifts
     # a cell face by ~1e-7 of a cell; low-mass cells next to such a face see up to ~1e-5)
-    exact = O.deposit_patches_f64(pat, bbox, 16)
+    exact = O.deposit_patches_ld(pat, bbox, 16)
     assert 0 < O.max_rel_err(exact, want) <= 1e-4
 
 
@@ -60,7 +64,7 @@ def test_boundary_deposit_tets_matches_fp64_oracle(n, rms, dims):
     bbox = A.cubic_root_box(lat.bbox())
     data = np.zeros(dims ** 3, np.float32)
     A.DepositWrapper.deposit_tets(True, 0, 0, pat.size // 1536, pat, pat.size, 7, bbox, dims, data)
-    want = O.deposit_patches_f64(pat, bbox, dims)
+    want = O.deposit_patches_ld(pat, bbox, dims)
     # the boundary returns float (the reference's grid type): one float rounding per cell
     assert O.max_rel_err(data, want) <= REL
     # accumulation contract: a second call adds
@@ -76,7 +80,7 @@ def test_boundary_patch_subrange_and_multipass():
     bbox = A.cubic_root_box(lat.bbox())
     dims = 20
     off, cnt = 2, npatch - 3
-    want = O.deposit_patches_f64(pat, bbox, dims, patch_offset=off, num_patches=cnt)
+    want = O.deposit_patches_ld(pat, bbox, dims, patch_offset=off, num_patches=cnt)
     one = np.zeros(dims ** 3, np.float32)
     A.DepositWrapper.deposit_tets(True, 0, off, cnt, pat, pat.size, 7, bbox, dims, one)
     assert O.max_rel_err(one, want) <= REL
@@ -116,7 +120,7 @@ def test_lattice_plain_matches_fp64_oracle(decomp, n, rms, dims):
     box = padded_box(lat.bbox())
     with A.ResidentLattice.from_lattice(lat) as rl:
         got, st = rl.deposit_grid(box, dims, decomposition=decomp, tet_mass=1.0 / 6, stats=True)
-    want = O.deposit_lattice(lat, box, dims, decomp=decomp, tet_mass=1.0 / 6)
+    want = O.deposit_lattice(lat, box, dims, decomp=decomp, tet_mass=1.0 / 6, precision=3)
     assert O.max_rel_err(got, want, tet_mass=1.0 / 6) <= REL
     total = lat.n ** 3  # 6 n^3 tets of mass 1/6, all inside the padded box
     assert abs(got.sum() - total) <= MASS_REL * total
@@ -130,7 +134,7 @@ def test_lattice_subset_and_bbox():
         ids = np.arange(0, lat.num_tets, 7, dtype=np.int32)
         box = A.cubic_root_box(lat.bbox())
         got = rl.deposit_grid(box, 16, tet_ids=ids)
-    want = O.deposit_lattice(lat, box, 16, tet_ids=ids)
+    want = O.deposit_lattice(lat, box, 16, tet_ids=ids, precision=3)
     assert O.max_rel_err(got, want) <= REL
 
 
@@ -142,5 +146,5 @@ def test_classification_regimes():
         coarse, st_c = rl.deposit_grid(box, 2, decomposition=A.DECOMP_KUHN, stats=True)
         fine, st_f = rl.deposit_grid(box, 96, decomposition=A.DECOMP_KUHN, stats=True)
     assert st_c[5] > 0 and st_f[6] > 0
-    assert O.max_rel_err(coarse, O.deposit_lattice(lat, box, 2, decomp=2)) <= REL
-    assert O.max_rel_err(fine, O.deposit_lattice(lat, box, 96, decomp=2)) <= REL
+    assert O.max_rel_err(coarse, O.deposit_lattice(lat, box, 2, decomp=2, precision=3)) <= REL
+    assert O.max_rel_err(fine, O.deposit_lattice(lat, box, 96, decomp=2, precision=3)) <= REL

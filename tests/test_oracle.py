@@ -106,6 +106,63 @@ def test_fp64_oracle_vs_reference_float_error_model(ref):
     assert abs(f.sum() - d.sum()) / d.sum() < 1e-6
 
 
+# ---------------------------------------------------------------- extended-precision evaluation
+def clipld(t):
+    t = np.ascontiguousarray(t, np.float64).reshape(4, 3)
+    return O.load().oracle_clip_unit_cube_ld(O.P(t))
+
+
+def test_extended_clip_closed_forms_and_rational():
+    for s in (0.05, 0.999, 1.0, 2.5, 4.0):
+        t = [[0, 0, 0], [0, s, 0], [s, 0, 0], [0, 0, s]]
+        assert clipld(t) == pytest.approx(irwin_hall(s), rel=1e-15, abs=1e-18)
+    rng = np.random.default_rng(31)
+    for _ in range(40):
+        v = oriented(rng.uniform(-0.8, 1.8, size=(4, 3)))
+        exact = tet_box_volume([[Fraction(x) for x in p] for p in v.tolist()])
+        assert clipld(v) == pytest.approx(float(exact), rel=1e-15, abs=1e-18)
+
+
+def test_extended_oracle_is_exact_where_fp64_cancels():
+    """The extended-precision oracle (the GPU parity checker) against exact rational volumes on
+    the cells where the FP64 evaluation of the reference's algorithm is worst: sliver cells that a
+    tet touches in a face or edge only, where FP64 cancellation leaves ~1e-13 of a tet mass."""
+    lat = make_lattice(16, 2.0, seed=7)
+    bbox = A.cubic_root_box(lat.bbox())
+    dims = 20
+    tets = O.lattice_tets(lat, 0)
+    with O.cell_scaling(O.EXACT):
+        ld = O.deposit_soup_ld(tets, bbox, dims)
+        d64 = O.deposit_soup_f64(tets, bbox, dims)
+    # the two agree everywhere to FP64 rounding of O(1) cell masses...
+    assert np.max(np.abs(ld - d64)) < 1e-10
+    # ...except relatively in near-empty cells: check the worst three against exact arithmetic
+    bad = np.argsort(-np.abs(ld - d64) / np.maximum(ld, 1e-8))[:3]
+    assert np.max(np.abs(ld - d64)[bad] / np.maximum(ld[bad], 1e-8)) > 1e-6
+    b = [Fraction(float(x)) for x in bbox]
+    mn, mx = tets.min(1), tets.max(1)
+    for c in bad:
+        c = int(c)
+        ijk = (c % dims, (c // dims) % dims, c // (dims * dims))
+        lo = [b[2 * a] + (b[2 * a + 1] - b[2 * a]) * Fraction(ijk[a], dims) for a in range(3)]
+        hi = [b[2 * a] + (b[2 * a + 1] - b[2 * a]) * Fraction(ijk[a] + 1, dims) for a in range(3)]
+        lof, hif = np.array([float(x) for x in lo]), np.array([float(x) for x in hi])
+        exact = Fraction(0)
+        for t in np.nonzero(np.all(mx >= lof - 1e-3, 1) & np.all(mn <= hif + 1e-3, 1))[0]:
+            v = [[Fraction(float(x)) for x in p] for p in tets[t].tolist()]
+            vol = abs(_det3_frac(v)) / 6
+            if vol:
+                exact += tet_box_volume(v, lo, hi) / vol
+        assert abs(ld[c] - float(exact)) <= 1e-15 * max(1.0, float(exact)), (c, ld[c], float(exact), d64[c])
+
+
+def _det3_frac(v):
+    a = [v[1][i] - v[0][i] for i in range(3)]
+    b = [v[2][i] - v[0][i] for i in range(3)]
+    c = [v[3][i] - v[0][i] for i in range(3)]
+    return a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) + a[2] * (b[0] * c[1] - b[1] * c[0])
+
+
 # ---------------------------------------------------------------- clip closed forms
 @pytest.mark.parametrize("s", [0.05, 0.5, 0.999, 1.0, 1.25, 2.0, 2.5, 2.99, 3.0, 4.0])
 def test_clip_irwin_hall(s):
