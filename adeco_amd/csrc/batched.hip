@@ -2,23 +2,27 @@
 //
 // The plain form gives every lane one tet and lets it walk its covered cells, so a wave runs as
 // long as its largest tet and leaves most lanes idle on heavy-tailed leaves.  Here the unit of
-// work is one (tet, candidate cell) pair:
-//   1. k_pair_setup     one lane per binned (grid, tet) pair: covered cell range in its grid
-//                       (AdDepositTets2Grid.h:514-539) -> candidate count;
+// work is one (tet, candidate cell) pair, and every wavefront works on its own, with no block
+// barriers:
+//   1. k_pair_setup     one lane per binned (grid, tet) pair: signed volume and orientation
+//                       (v0 <-> v3 when negative, AdDepositTets2Grid.h:547-558), vertices scaled to
+//                       the grid's cell units in FP64, mass factor, covered cell range
+//                       (AdDepositTets2Grid.h:514-539) -> one 128-byte PairData line and a
+//                       candidate count;
 //   2. scan             candidate counts -> 64-bit offsets, total C;
-//   3. k_tile_map       first pair of every tile of kTile consecutive candidates;
-//   4. process_tile     per tile: the tile's pairs (<= kTile + 1) are staged in LDS once (vertices
-//                       scaled to the grid's cell units in FP64, mass factor, covered range); each
-//                       lane binary-searches LDS for its candidate's pair, culls by AABB and
-//                       classifies the cell with the 4 face planes evaluated at the extreme corners
-//                       (a relative margin of 1e-12: an uncertain cell is clipped, never skipped or
-//                       filled); trivial cells are deposited at once and partial cells are
-//                       compacted into an LDS queue that all lanes then clip, so the exact clip runs
-//                       with full waves.
-// k_deposit_tiles (octree leaves) pulls the tiles of one candidate space from an atomic work
-// counter; k_deposit_step (the request server) pulls (space, tile) entries spanning several
-// candidate spaces, so requests of different sizes share one launch.  Masses go to the grids with
-// FP64 global atomics.  Precision modes for the clip:
+//   3. k_tile_map       first pair of every tile of 64 consecutive candidates;
+//   4. k_deposit_waves  persistent wavefronts pop chunks of tiles from a work-queue counter.  Per
+//                       tile each lane finds its candidate's pair (a marker per pair start and a
+//                       wave prefix-max), culls the cell by AABB and classifies it with the tet's
+//                       4 face planes at the extreme cell corners (a relative margin of 1e-12: an
+//                       uncertain cell is clipped, never skipped or filled).  Cells inside the tet
+//                       or holding the whole tet are deposited at once; partial cells go to one of
+//                       9 per-wave LDS queues keyed by their clip work class (number of x/y faces
+//                       that cut the tet once the axis with the most cutting faces is taken as the
+//                       closed-form z axis, and whether the first cut leaves 1 or 3 sub-tets).  A
+//                       queue that reaches 64 entries is clipped by the whole wave, so the lanes of
+//                       one clip follow nearly the same path through the nested clip.
+// Masses go to the grids with FP64 global atomics.  Precision modes for the clip:
 //   FP64  all FP64;
 //   MIXED FP32 clip on cell-local coordinates formed in FP64, accepted only when its error bound
 //         16 * eps32 * max(1, M)^2 (M = largest |coordinate| of the cell-local tet; twice the worst
@@ -54,23 +58,25 @@ GridDesc make_grid_desc(const GridGeom& g, long long base) {
     return d;
 }
 
+LaunchShape& launch_shape() {
+    static LaunchShape s;
+    return s;
+}
+
 namespace {
 
-struct PairLds {
-    double S[12];  // scale_a * (src_q - gmin), oriented vertex order
-    double mcell;  // tet_mass * cell_vol / |V|
-    int grid;
-    unsigned short lo[3], ext[3];  // covered cells: lo .. lo + ext - 1 (ext = 0: dead pair)
-};
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = 64 * kWavesPerBlock;
+constexpr int kBuckets = 9;        // clip work classes
+constexpr int kQueueCap = 2 * kWaveTile;
 
-struct TileShared {
-    PairLds pair[kTile + 1];
-    int64_t off[kTile + 2];
-    unsigned short qslot[kTile];
-    unsigned qr[kTile];
-    int qn, np;
-    int64_t tile;
-    int slot;
+// Per-wavefront LDS: pair-start markers of the current tile and the bucketed partial-cell queues.
+// A queue entry is (pair, cell | closed-form axis << 30) plus the candidate space it belongs to.
+struct WaveLds {
+    int mark[kWaveTile];
+    int qn[16];
+    int2 q[kBuckets][kQueueCap];
+    unsigned char qs[kBuckets][kQueueCap];
 };
 
 __device__ __forceinline__ int upper_bound_i64(const int64_t* a, int n, int64_t v) {
@@ -83,12 +89,12 @@ __device__ __forceinline__ int upper_bound_i64(const int64_t* a, int n, int64_t 
     return lo;
 }
 
-// per pair: grid, covered range, candidate count (dead pairs keep one dummy candidate so every
-// tile spans at most kTile + 1 pairs)
+// per pair: PairData and candidate count (dead pairs keep one dummy candidate so every pair owns
+// at least one candidate and a tile of 64 candidates spans at most 64 pairs)
 __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorners ct, const int32_t* __restrict__ pairs,
                                                     int64_t n_pairs, const int64_t* __restrict__ grid_off, int n_grids,
-                                                    const GridDesc* __restrict__ grids, unsigned* __restrict__ cand,
-                                                    unsigned long long* __restrict__ range, int32_t* __restrict__ pair_grid) {
+                                                    const GridDesc* __restrict__ grids, double tet_mass,
+                                                    unsigned* __restrict__ cand, PairData* __restrict__ pd) {
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n_pairs) return;
     const int gi = upper_bound_i64(grid_off, n_grids + 1, p) - 1;
@@ -104,7 +110,7 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
         }
     }
     unsigned long long pack = 0;
-    const double vol = vol6(v[0], v[1], v[2], v[3]);
+    const double vol = vol6(v[0], v[1], v[2], v[3]) / 6.0;
     const int dm1 = g.dims - 1;
     unsigned n = 1;
     for (int a = 0; a < 3; ++a) {
@@ -120,9 +126,22 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
         pack |= (unsigned long long)e << (30 + 10 * a);
     }
     const bool live = vol != 0.0;
+    if (vol < 0.0) {
+        const V3<double> t = v[0];
+        v[0] = v[3];
+        v[3] = t;
+    }
+    PairData& o = pd[p];
+    for (int w = 0; w < 4; ++w) {
+        o.S[3 * w] = g.scale[0] * (v[w].x - g.gmin[0]);
+        o.S[3 * w + 1] = g.scale[1] * (v[w].y - g.gmin[1]);
+        o.S[3 * w + 2] = g.scale[2] * (v[w].z - g.gmin[2]);
+    }
+    o.mcell = live ? tet_mass * (g.cell_vol / fabs(vol)) : 0.0;
+    o.range = live ? pack : 0ull;  // extent 0 marks a dead pair
+    o.grid = gi;
+    o.pad_[0] = o.pad_[1] = o.pad_[2] = 0;
     cand[p] = live ? n : 1u;
-    range[p] = live ? pack : 0ull;  // ext = 0 marks a dead pair
-    pair_grid[p] = gi;
 }
 
 // exclusive scan of uint32 counts into int64 offsets: block-local pass, block-sum pass, add pass
@@ -186,14 +205,23 @@ __global__ void __launch_bounds__(256) k_tile_map(const int64_t* __restrict__ of
         tile_first[n_tiles] = int32_t(n_pairs - 1);
         return;
     }
-    const int64_t t0 = (off[p] + kTile - 1) / kTile, t1 = (off[p + 1] + kTile - 1) / kTile;
+    const int64_t t0 = (off[p] + kWaveTile - 1) / kWaveTile, t1 = (off[p + 1] + kWaveTile - 1) / kWaveTile;
     for (int64_t t = t0; t < t1 && t < n_tiles; ++t) tile_first[t] = int32_t(p);
 }
 
 template <class R>
-__device__ __forceinline__ void load_s(const PairLds& pr, const GridDesc& g, int i, int j, int k, V3<R> s[4]) {
+__device__ __forceinline__ void cell_local(const PairData& pd, const GridDesc& g, int i, int j, int k, V3<R> s[4]) {
     const double ox = double(i) * g.r[0], oy = double(j) * g.r[1], oz = double(k) * g.r[2];
-    for (int q = 0; q < 4; ++q) s[q] = mk(R(pr.S[3 * q] - ox), R(pr.S[3 * q + 1] - oy), R(pr.S[3 * q + 2] - oz));
+    for (int q = 0; q < 4; ++q) s[q] = mk(R(pd.S[3 * q] - ox), R(pd.S[3 * q + 1] - oy), R(pd.S[3 * q + 2] - oz));
+}
+
+// the axis given to the closed-form slab integration becomes z
+template <class R>
+__device__ __forceinline__ void rotate_axes(V3<R> s[4], int axis) {
+    for (int q = 0; q < 4; ++q) {
+        const V3<R> u = s[q];
+        s[q] = axis == 0 ? mk(u.y, u.z, u.x) : (axis == 1 ? mk(u.z, u.x, u.y) : u);
+    }
 }
 
 // Conservative classification of the cell-local tet s against [0,r] (FP64).
@@ -225,66 +253,82 @@ __device__ __forceinline__ int classify_margin(const V3<double> s[4], const doub
     return tin ? CLS_TET_IN_CELL : CLS_PARTIAL;
 }
 
-template <int kPrec>
-__device__ __forceinline__ void process_tile(const TileSource& src, int64_t t, const LatticeView& lat,
-                                             const PackedCorners& ct, TileShared& sh,
-                                             unsigned long long& my_fallback, unsigned long long* counters) {
-    const int tid = threadIdx.x;
+// Clip work class of a partial cell: the axis with the most faces cutting the tet is integrated in
+// closed form (ties -> z); the class counts the remaining x/y faces that cut and whether the
+// first cut leaves one sub-tet or a prism.
+__device__ __forceinline__ int work_class(const V3<double> s[4], const double mn[3], const double mx[3], int& axis) {
+    int act[3];
+    for (int a = 0; a < 3; ++a) act[a] = int(mn[a] < 0.0) + int(mx[a] > 1.0);
+    axis = 2;
+    if (act[0] > act[axis]) axis = 0;
+    if (act[1] > act[axis]) axis = 1;
+    V3<double> u[4] = {s[0], s[1], s[2], s[3]};
+    rotate_axes(u, axis);
+    const int faces = active_xy_faces(u);
+    if (faces <= 0) return 0;
+    const int nact = (faces & 1) + ((faces >> 1) & 1) + ((faces >> 2) & 1) + ((faces >> 3) & 1);
+    const int f = __builtin_ctz(unsigned(faces));
+    const int ax = f >> 1;
+    const bool ge = (f & 1) == 0;
+    int nin = 0;
+    for (int q = 0; q < 4; ++q) {
+        const double x = comp(u[q], ax);
+        nin += int(ge ? x > 0.0 : 1.0 - x > 0.0);
+    }
+    return 2 * nact - 1 + int(nin > 1);
+}
+
+__device__ __forceinline__ unsigned lane_mask_lt(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u));
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int64_t shfl_i64(int64_t v, int src) {
+    const int lo = __shfl(int(uint64_t(v) & 0xffffffffu), src, 64);
+    const int hi = __shfl(int(uint64_t(v) >> 32), src, 64);
+    return int64_t((uint64_t(unsigned(hi)) << 32) | unsigned(lo));
+}
+
+// Classifies the 64 candidates of tile t of `src` (slot `slot` among the launch's sources):
+// trivial cells are deposited, partial cells are appended to the wave's queues.
+__device__ __forceinline__ void classify_tile(const TileSource& src, int slot, int64_t t, WaveLds& L, int lane,
+                                              unsigned long long& n_partial) {
     const int p0 = src.tile_first[t];
-    const int p1 = src.tile_first[t + 1];
-    const int np = p1 - p0 + 1;  // <= kTile + 1 (every pair holds >= 1 candidate)
-    for (int q = tid; q < np; q += kTile) {
-        const int p = p0 + q;
-        PairLds pr;
-        const unsigned long long pk = src.range[p];
-        pr.grid = src.pair_grid[p];
-        for (int a = 0; a < 3; ++a) {
-            pr.lo[a] = (unsigned short)((pk >> (10 * a)) & 1023u);
-            pr.ext[a] = (unsigned short)((pk >> (30 + 10 * a)) & 1023u);
-        }
-        if (pr.ext[0]) {
-            V3<double> v[4];
-            lattice_tet_vertices(lat, ct, src.pairs[p], v);
-            const double vol = vol6(v[0], v[1], v[2], v[3]) / 6.0;
-            if (vol < 0.0) {
-                const V3<double> tmp = v[0];
-                v[0] = v[3];
-                v[3] = tmp;
-            }
-            const GridDesc& g = src.grids[pr.grid];
-            for (int w = 0; w < 4; ++w) {
-                pr.S[3 * w] = g.scale[0] * (v[w].x - g.gmin[0]);
-                pr.S[3 * w + 1] = g.scale[1] * (v[w].y - g.gmin[1]);
-                pr.S[3 * w + 2] = g.scale[2] * (v[w].z - g.gmin[2]);
-            }
-            pr.mcell = src.tet_mass * (g.cell_vol / fabs(vol));
-        } else {
-            for (int w = 0; w < 12; ++w) pr.S[w] = 0.0;
-            pr.mcell = 0.0;
-        }
-        sh.pair[q] = pr;
-        sh.off[q] = src.cand_off[p];
+    const int np = min(src.tile_first[t + 1] - p0 + 1, kWaveTile);
+    const int64_t c0 = t * kWaveTile;
+    const int64_t my_off = lane < np ? src.cand_off[p0 + lane] : int64_t(0);
+    L.mark[lane] = 0;
+    wave_sync();
+    // every pair owns >= 1 candidate, so pairs p0+1.. start at distinct positions 1..63
+    if (lane >= 1 && lane < np && my_off - c0 < kWaveTile) L.mark[int(my_off - c0)] = lane;
+    wave_sync();
+    int slot_in_tile = L.mark[lane];
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(slot_in_tile, d, 64);
+        if (lane >= d) slot_in_tile = max(slot_in_tile, o);
     }
-    if (tid == 0) {
-        sh.np = np;
-        sh.off[np] = src.cand_off[p0 + np];
-    }
-    __syncthreads();
-    // classify one candidate per lane
-    const int64_t c = t * kTile + tid;
+    const int64_t off = shfl_i64(my_off, slot_in_tile);
+    const int64_t c = c0 + lane;
+    const int p = p0 + slot_in_tile;
+    int key = -1, cell = 0;
     if (c < src.n_cand) {
-        const int slot = upper_bound_i64(sh.off, sh.np, c) - 1;
-        const PairLds& pr = sh.pair[slot];
-        if (pr.ext[0]) {
-            const unsigned r = unsigned(c - sh.off[slot]);
-            const unsigned ex = pr.ext[0], ey = pr.ext[1];
-            const int i = pr.lo[0] + int(r % ex);
-            const int j = pr.lo[1] + int((r / ex) % ey);
-            const int k = pr.lo[2] + int(r / (ex * ey));
-            const GridDesc& g = src.grids[pr.grid];
+        const PairData& pd = src.pd[p];
+        const unsigned long long rg = pd.range;
+        const unsigned ex = unsigned(rg >> 30) & 1023u;
+        if (ex) {
+            const unsigned ey = unsigned(rg >> 40) & 1023u;
+            const unsigned r = unsigned(c - off);
+            const int i = int(rg & 1023u) + int(r % ex);
+            const int j = int((rg >> 10) & 1023u) + int((r / ex) % ey);
+            const int k = int((rg >> 20) & 1023u) + int(r / (ex * ey));
+            const GridDesc& g = src.grids[pd.grid];
             V3<double> s[4];
-            load_s(pr, g, i, j, k, s);
-            // strict AABB cull against the cell [0, r] (cell units)
+            cell_local(pd, g, i, j, k, s);
             double mn[3] = {s[0].x, s[0].y, s[0].z}, mx[3] = {s[0].x, s[0].y, s[0].z};
             for (int q = 1; q < 4; ++q) {
                 const double cc[3] = {s[q].x, s[q].y, s[q].z};
@@ -293,128 +337,195 @@ __device__ __forceinline__ void process_tile(const TileSource& src, int64_t t, c
                     mx[a] = cc[a] > mx[a] ? cc[a] : mx[a];
                 }
             }
-            const bool hit = mn[0] < g.r[0] && 0.0 < mx[0] && mn[1] < g.r[1] && 0.0 < mx[1] && mn[2] < g.r[2] && 0.0 < mx[2];
+            // strict AABB cull against the cell box [0, r]; the reference then classifies and
+            // clips against the unit cube in scaled coordinates (AdDepositTets2Grid.h:587-634)
+            const bool hit = mn[0] < g.r[0] && 0.0 < mx[0] && mn[1] < g.r[1] && 0.0 < mx[1] && mn[2] < g.r[2] &&
+                             0.0 < mx[2];
             if (hit) {
-                // the reference culls against the cell box but classifies and clips against the
-                // unit cube in scaled coordinates (AdDepositTets2Grid.h:587-634)
                 const double unit[3] = {1.0, 1.0, 1.0};
                 const int cls = classify_margin(s, unit);
                 const int64_t idx = g.base + i + int64_t(g.dims) * (j + int64_t(g.dims) * k);
                 if (cls == CLS_TET_IN_CELL) {
                     unsafeAtomicAdd(src.grid_buf + idx, src.tet_mass);
                 } else if (cls == CLS_CELL_IN_TET) {
-                    unsafeAtomicAdd(src.grid_buf + idx, pr.mcell);
+                    unsafeAtomicAdd(src.grid_buf + idx, pd.mcell);
                 } else if (cls == CLS_PARTIAL) {
-                    const int qi = atomicAdd(&sh.qn, 1);
-                    sh.qslot[qi] = (unsigned short)slot;
-                    sh.qr[qi] = r;
+                    int axis;
+                    key = work_class(s, mn, mx, axis);
+                    cell = int(unsigned(i) | (unsigned(j) << 10) | (unsigned(k) << 20) | (unsigned(axis) << 30));
                 }
             }
         }
     }
-    __syncthreads();
-    // clip the compacted partial cells with all lanes
-    const int nq = sh.qn;
-    if (tid == 0 && nq) atomicAdd(counters + 1, (unsigned long long)nq);
-    for (int qi = tid; qi < nq; qi += kTile) {
-        const PairLds& pr = sh.pair[sh.qslot[qi]];
-        const unsigned r = sh.qr[qi];
-        const unsigned ex = pr.ext[0], ey = pr.ext[1];
-        const int i = pr.lo[0] + int(r % ex);
-        const int j = pr.lo[1] + int((r / ex) % ey);
-        const int k = pr.lo[2] + int(r / (ex * ey));
-        const GridDesc& g = src.grids[pr.grid];
-        double w6;
-        if (kPrec == ADECO_PREC_FP64) {
-            V3<double> s[4];
-            load_s(pr, g, i, j, k, s);
-            w6 = clip_unit_cube_vol6(s);
-        } else {
-            V3<float> s[4];
-            load_s(pr, g, i, j, k, s);
-            const float w32 = clip_unit_cube_vol6(s);
-            w6 = double(w32);
-            if (kPrec == ADECO_PREC_MIXED) {
-                float M = 1.f;
-                for (int q = 0; q < 4; ++q) M = fmaxf(M, fmaxf(fmaxf(fabsf(s[q].x), fabsf(s[q].y)), fabsf(s[q].z)));
-                const float bound = 16.f * 5.9604645e-8f * M * M;
-                if (!(bound <= 2.5e-7f * w32)) {
-                    V3<double> sd[4];
-                    load_s(pr, g, i, j, k, sd);
-                    w6 = clip_unit_cube_vol6(sd);
-                    ++my_fallback;
-                }
+    // append the partial cells to their queues (every queue holds < 64 here, so < 128 after)
+    unsigned long long any = __ballot(key >= 0);
+    while (any) {
+        const int b = __builtin_amdgcn_readfirstlane(__shfl(key, __builtin_ctzll(any), 64));
+        const unsigned long long m = __ballot(key == b);
+        const int base = L.qn[b];
+        if (key == b) {
+            const int pos = base + int(lane_mask_lt(m));
+            L.q[b][pos] = make_int2(p, cell);
+            L.qs[b][pos] = (unsigned char)slot;
+        }
+        wave_sync();
+        if (lane == 0) L.qn[b] = base + __popcll(m);
+        wave_sync();
+        n_partial += __popcll(m);
+        any &= ~m;
+    }
+}
+
+template <int kPrec, class Sources>
+__device__ __forceinline__ void clip_entry(const Sources& srcs, int2 e, int slot, unsigned long long& fb) {
+    const TileSource& src = srcs.get(slot);
+    const PairData& pd = src.pd[e.x];
+    const unsigned u = unsigned(e.y);
+    const int i = int(u & 1023u), j = int((u >> 10) & 1023u), k = int((u >> 20) & 1023u), axis = int(u >> 30);
+    const GridDesc& g = src.grids[pd.grid];
+    double w6;
+    if (kPrec == ADECO_PREC_FP64) {
+        V3<double> s[4];
+        cell_local(pd, g, i, j, k, s);
+        rotate_axes(s, axis);
+        w6 = clip_unit_cube_vol6(s);
+    } else {
+        V3<float> s[4];
+        cell_local(pd, g, i, j, k, s);
+        rotate_axes(s, axis);
+        const float w32 = clip_unit_cube_vol6(s);
+        w6 = double(w32);
+        if (kPrec == ADECO_PREC_MIXED) {
+            float M = 1.f;
+            for (int q = 0; q < 4; ++q) M = fmaxf(M, fmaxf(fmaxf(fabsf(s[q].x), fabsf(s[q].y)), fabsf(s[q].z)));
+            const float bound = 16.f * 5.9604645e-8f * M * M;
+            if (!(bound <= 2.5e-7f * w32)) {
+                V3<double> sd[4];
+                cell_local(pd, g, i, j, k, sd);
+                rotate_axes(sd, axis);
+                w6 = clip_unit_cube_vol6(sd);
+                ++fb;
             }
         }
-        const double dm = pr.mcell * (w6 * (1.0 / 6.0));
-        if (dm > 0.0) unsafeAtomicAdd(src.grid_buf + g.base + i + int64_t(g.dims) * (j + int64_t(g.dims) * k), dm);
     }
-    __syncthreads();
+    const double dm = pd.mcell * (w6 * (1.0 / 6.0));
+    if (dm > 0.0) unsafeAtomicAdd(src.grid_buf + g.base + i + int64_t(g.dims) * (j + int64_t(g.dims) * k), dm);
+}
+
+struct OneSource {
+    TileSource s;
+    __device__ const TileSource& get(int) const { return s; }
+};
+struct ManySources {
+    const TileSource* s;
+    __device__ const TileSource& get(int i) const { return s[i]; }
+};
+
+// The persistent loop of one wavefront over work items [0, n_items): item -> (source slot, tile)
+// directly (one source) or through `entries`.  The wave pops `chunk` items per work-queue access;
+// a queue reaching 64 entries is clipped at once; when the items run out the queues are drained.
+// Every wave leaves the loop once the counter passes n_items and its queues are empty.
+template <int kPrec, class Sources>
+__device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __restrict__ entries, int64_t n_items,
+                                          int chunk, unsigned* __restrict__ queue_counter,
+                                          unsigned long long* __restrict__ counters) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    const int lane = threadIdx.x & 63;
+    WaveLds& L = lds[threadIdx.x >> 6];
+    if (lane < 16) L.qn[lane] = 0;
+    wave_sync();
+    unsigned long long n_partial = 0, fb = 0;
+    int64_t it = 0, it_end = 0;
+    bool draining = false;
+    for (;;) {
+        // a queue to clip: full (64) while tiles remain, any non-empty one when draining
+        const int need = draining ? 1 : kWaveTile;
+        const int qv = lane < kBuckets ? L.qn[lane] : 0;
+        const unsigned long long ready = __ballot(qv >= need);
+        if (ready) {
+            const int b = __builtin_ctzll(ready);
+            const int cnt = __builtin_amdgcn_readfirstlane(__shfl(qv, b, 64));
+            const int take = min(cnt, kWaveTile);
+            const int rest = cnt - take;
+            int2 e = make_int2(0, 0);
+            int slot = 0;
+            if (lane < take) {
+                e = L.q[b][rest + lane];
+                slot = L.qs[b][rest + lane];
+            }
+            wave_sync();
+            if (lane == 0) L.qn[b] = rest;
+            wave_sync();
+            if (lane < take) clip_entry<kPrec>(srcs, e, slot, fb);
+            continue;
+        }
+        if (draining) break;
+        if (it == it_end) {
+            unsigned base = 0;
+            if (lane == 0) base = atomicAdd(queue_counter, 1u);
+            base = __builtin_amdgcn_readfirstlane(base);
+            it = int64_t(base) * chunk;
+            it_end = min(it + int64_t(chunk), n_items);
+            if (it >= n_items) {
+                draining = true;
+                continue;
+            }
+        }
+        int slot = 0;
+        int64_t t = it;
+        if (entries) {
+            const int2 en = entries[it];
+            slot = __builtin_amdgcn_readfirstlane(en.x);
+            t = int64_t(__builtin_amdgcn_readfirstlane(en.y));
+        }
+        ++it;
+        classify_tile(srcs.get(slot), slot, t, L, lane, n_partial);
+    }
+    if (lane == 0 && n_partial) atomicAdd(counters + 1, n_partial);
+    if (kPrec == ADECO_PREC_MIXED && fb) atomicAdd(counters, fb);
 }
 
 template <int kPrec>
-__global__ void __launch_bounds__(kTile) k_deposit_tiles(TileSource src, LatticeView lat, PackedCorners ct,
-                                                         unsigned* __restrict__ tile_counter,
+__global__ void __launch_bounds__(kBlock) k_deposit_waves(TileSource src, int chunk, unsigned* __restrict__ queue_counter,
+                                                          unsigned long long* __restrict__ counters) {
+    wave_loop<kPrec>(OneSource{src}, nullptr, src.n_tiles, chunk, queue_counter, counters);
+}
+
+template <int kPrec>
+__global__ void __launch_bounds__(kBlock) k_deposit_step(const TileSource* __restrict__ sources,
+                                                         const int2* __restrict__ entries, int n_entries, int chunk,
+                                                         unsigned* __restrict__ queue_counter,
                                                          unsigned long long* __restrict__ counters) {
-    __shared__ TileShared sh;
-    unsigned long long my_fallback = 0;
-    for (;;) {
-        if (threadIdx.x == 0) {
-            sh.tile = int64_t(atomicAdd(tile_counter, 1u));
-            sh.qn = 0;
-        }
-        __syncthreads();
-        const int64_t t = sh.tile;
-        if (t >= src.n_tiles) break;
-        process_tile<kPrec>(src, t, lat, ct, sh, my_fallback, counters);
-    }
-    if (kPrec == ADECO_PREC_MIXED && my_fallback) atomicAdd(counters, my_fallback);
-}
-
-template <int kPrec>
-__global__ void __launch_bounds__(kTile) k_deposit_step(const TileSource* __restrict__ sources, const int2* __restrict__ entries,
-                                                        int n_entries, LatticeView lat, PackedCorners ct,
-                                                        unsigned* __restrict__ entry_counter,
-                                                        unsigned long long* __restrict__ counters) {
-    __shared__ TileShared sh;
-    unsigned long long my_fallback = 0;
-    for (;;) {
-        if (threadIdx.x == 0) {
-            sh.tile = int64_t(atomicAdd(entry_counter, 1u));
-            sh.qn = 0;
-        }
-        __syncthreads();
-        const int64_t e = sh.tile;
-        if (e >= n_entries) break;
-        const int2 en = entries[e];
-        process_tile<kPrec>(sources[en.x], int64_t(en.y), lat, ct, sh, my_fallback, counters);
-    }
-    if (kPrec == ADECO_PREC_MIXED && my_fallback) atomicAdd(counters, my_fallback);
+    wave_loop<kPrec>(ManySources{sources}, entries, n_entries, chunk, queue_counter, counters);
 }
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+int resident_blocks(int device) {
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    return cus * std::max(1, launch_shape().blocks_per_cu);
+}
 
 }  // namespace
 
 size_t candidate_space_bytes(int64_t n_pairs) {
     const int64_t nblk = (n_pairs + kScanBlock - 1) / kScanBlock;
-    return align256(n_pairs * 4) + align256(n_pairs * 4) + align256(n_pairs * 8) + align256((n_pairs + 1) * 8) +
+    return align256(size_t(n_pairs) * sizeof(PairData)) + align256(n_pairs * 4) + align256((n_pairs + 1) * 8) +
            align256(nblk * 8 + 8) + align256(8);
 }
 
-size_t tile_map_bytes(int64_t n_cand) { return align256(size_t((n_cand + kTile - 1) / kTile + 1) * 4); }
+size_t tile_map_bytes(int64_t n_cand) { return align256(size_t((n_cand + kWaveTile - 1) / kWaveTile + 1) * 4); }
 
 adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, const PackedCorners& ct,
                                    const int32_t* pairs, int64_t n_pairs, const int64_t* grid_off, int n_grids,
-                                   const GridDesc* d_grids, void* slab, CandidateSpace& cs) {
+                                   const GridDesc* d_grids, double tet_mass, void* slab, CandidateSpace& cs) {
     char* w = static_cast<char*>(slab);
     const int64_t nblk = (n_pairs + kScanBlock - 1) / kScanBlock;
-    cs.pair_grid = reinterpret_cast<int32_t*>(w);
-    w += align256(n_pairs * 4);
+    cs.pd = reinterpret_cast<PairData*>(w);
+    w += align256(size_t(n_pairs) * sizeof(PairData));
     cs.cand = reinterpret_cast<unsigned*>(w);
     w += align256(n_pairs * 4);
-    cs.range = reinterpret_cast<unsigned long long*>(w);
-    w += align256(n_pairs * 8);
     cs.cand_off = reinterpret_cast<int64_t*>(w);
     w += align256((n_pairs + 1) * 8);
     cs.block_sum = reinterpret_cast<int64_t*>(w);
@@ -425,7 +536,7 @@ adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, c
     cs.n_tiles = 0;
     if (n_pairs == 0) return ADECO_OK;
     hipLaunchKernelGGL(k_pair_setup, dim3(unsigned((n_pairs + 255) / 256)), dim3(256), 0, stream, lat, ct, pairs, n_pairs,
-                       grid_off, n_grids, d_grids, cs.cand, cs.range, cs.pair_grid);
+                       grid_off, n_grids, d_grids, tet_mass, cs.cand, cs.pd);
     hipLaunchKernelGGL(k_scan_local, dim3(unsigned(nblk)), dim3(256), 0, stream, cs.cand, n_pairs, cs.cand_off, cs.block_sum);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, stream, cs.block_sum, nblk, cs.total);
     hipLaunchKernelGGL(k_scan_add, dim3(unsigned((n_pairs + 1 + 255) / 256)), dim3(256), 0, stream, cs.cand_off, n_pairs,
@@ -434,7 +545,7 @@ adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, c
     if (e == hipSuccess) e = hipMemcpyAsync(&cs.n_cand, cs.total, 8, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "candidate space setup");
-    cs.n_tiles = (cs.n_cand + kTile - 1) / kTile;
+    cs.n_tiles = (cs.n_cand + kWaveTile - 1) / kWaveTile;
     return ADECO_OK;
 }
 
@@ -447,41 +558,39 @@ adeco_status build_tile_map(hipStream_t stream, CandidateSpace& cs, int32_t* til
     return e == hipSuccess ? ADECO_OK : hip_fail(e, "tile map");
 }
 
-hipError_t launch_deposit_tiles(hipStream_t stream, int precision, const TileSource& src, const LatticeView& lat,
-                                const PackedCorners& ct, unsigned* tile_counter, unsigned long long* counters,
-                                int grid_blocks) {
+hipError_t launch_deposit_tiles(hipStream_t stream, int precision, const TileSource& src, unsigned* queue_counter,
+                                unsigned long long* counters, int device) {
     if (src.n_tiles <= 0) return hipSuccess;
-    const unsigned grid = unsigned(std::min<int64_t>(src.n_tiles, grid_blocks));
+    const int chunk = std::max(1, launch_shape().chunk);
+    const int64_t waves_needed = (src.n_tiles + chunk - 1) / chunk;
+    const unsigned grid = unsigned(std::max<int64_t>(1, std::min<int64_t>((waves_needed + kWavesPerBlock - 1) / kWavesPerBlock,
+                                                                          resident_blocks(device))));
     if (precision == ADECO_PREC_FP64)
-        hipLaunchKernelGGL(k_deposit_tiles<ADECO_PREC_FP64>, dim3(grid), dim3(kTile), 0, stream, src, lat, ct, tile_counter, counters);
+        hipLaunchKernelGGL(k_deposit_waves<ADECO_PREC_FP64>, dim3(grid), dim3(kBlock), 0, stream, src, chunk, queue_counter, counters);
     else if (precision == ADECO_PREC_MIXED)
-        hipLaunchKernelGGL(k_deposit_tiles<ADECO_PREC_MIXED>, dim3(grid), dim3(kTile), 0, stream, src, lat, ct, tile_counter, counters);
+        hipLaunchKernelGGL(k_deposit_waves<ADECO_PREC_MIXED>, dim3(grid), dim3(kBlock), 0, stream, src, chunk, queue_counter, counters);
     else
-        hipLaunchKernelGGL(k_deposit_tiles<ADECO_PREC_FP32>, dim3(grid), dim3(kTile), 0, stream, src, lat, ct, tile_counter, counters);
+        hipLaunchKernelGGL(k_deposit_waves<ADECO_PREC_FP32>, dim3(grid), dim3(kBlock), 0, stream, src, chunk, queue_counter, counters);
     return hipGetLastError();
 }
 
 hipError_t launch_deposit_step(hipStream_t stream, int precision, const TileSource* d_sources, const int2* d_entries,
-                               int n_entries, const LatticeView& lat, const PackedCorners& ct, unsigned* entry_counter,
-                               unsigned long long* counters, int grid_blocks) {
+                               int n_entries, unsigned* queue_counter, unsigned long long* counters, int device) {
     if (n_entries <= 0) return hipSuccess;
-    const unsigned grid = unsigned(std::min<int64_t>(n_entries, grid_blocks));
+    const int chunk = std::max(1, launch_shape().chunk);
+    const int64_t waves_needed = (int64_t(n_entries) + chunk - 1) / chunk;
+    const unsigned grid = unsigned(std::max<int64_t>(1, std::min<int64_t>((waves_needed + kWavesPerBlock - 1) / kWavesPerBlock,
+                                                                          resident_blocks(device))));
     if (precision == ADECO_PREC_FP64)
-        hipLaunchKernelGGL(k_deposit_step<ADECO_PREC_FP64>, dim3(grid), dim3(kTile), 0, stream, d_sources, d_entries, n_entries,
-                           lat, ct, entry_counter, counters);
+        hipLaunchKernelGGL(k_deposit_step<ADECO_PREC_FP64>, dim3(grid), dim3(kBlock), 0, stream, d_sources, d_entries, n_entries,
+                           chunk, queue_counter, counters);
     else if (precision == ADECO_PREC_MIXED)
-        hipLaunchKernelGGL(k_deposit_step<ADECO_PREC_MIXED>, dim3(grid), dim3(kTile), 0, stream, d_sources, d_entries, n_entries,
-                           lat, ct, entry_counter, counters);
+        hipLaunchKernelGGL(k_deposit_step<ADECO_PREC_MIXED>, dim3(grid), dim3(kBlock), 0, stream, d_sources, d_entries, n_entries,
+                           chunk, queue_counter, counters);
     else
-        hipLaunchKernelGGL(k_deposit_step<ADECO_PREC_FP32>, dim3(grid), dim3(kTile), 0, stream, d_sources, d_entries, n_entries,
-                           lat, ct, entry_counter, counters);
+        hipLaunchKernelGGL(k_deposit_step<ADECO_PREC_FP32>, dim3(grid), dim3(kBlock), 0, stream, d_sources, d_entries, n_entries,
+                           chunk, queue_counter, counters);
     return hipGetLastError();
-}
-
-int persistent_blocks(int device) {
-    int cus = 256;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-    return cus * 4;
 }
 
 // Leaf deposit of an octree: one candidate space over all leaves, one persistent launch.
@@ -507,7 +616,7 @@ adeco_status deposit_leaves_batched(adeco_octree_s* T, int dims, int decompositi
     const PackedCorners ct = pack_corners(corner_table(decomposition));
     CandidateSpace cs;
     if ((st = build_candidate_space(ds.stream, lv, ct, T->d_pairs.as<int32_t>(), P, T->d_leaf_offset.as<int64_t>(), nl,
-                                    T->d_leaf_geom.as<GridDesc>(), T->d_work.ptr, cs)))
+                                    T->d_leaf_geom.as<GridDesc>(), tet_mass, T->d_work.ptr, cs)))
         return st;
     if ((st = T->d_tiles.reserve(tile_map_bytes(cs.n_cand)))) return st;
     if ((st = build_tile_map(ds.stream, cs, T->d_tiles.as<int32_t>()))) return st;
@@ -515,10 +624,10 @@ adeco_status deposit_leaves_batched(adeco_octree_s* T, int dims, int decompositi
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(T->d_counters.as<char>() + 16);
     e = hipMemsetAsync(T->d_counters.ptr, 0, 64, ds.stream);
     if (e != hipSuccess) return hip_fail(e, "batched counters");
-    const TileSource src{T->d_pairs.as<int32_t>(), cs.pair_grid, cs.range, cs.cand_off, cs.tile_first, cs.n_tiles,
-                         cs.n_cand, T->d_leaf_geom.as<GridDesc>(), T->d_grids.as<double>(), tet_mass};
+    const TileSource src{cs.pd, cs.cand_off, cs.tile_first, cs.n_tiles, cs.n_cand, T->d_leaf_geom.as<GridDesc>(),
+                         T->d_grids.as<double>(), tet_mass};
     e = hipEventRecord(ds.evk0, ds.stream);
-    if (e == hipSuccess) e = launch_deposit_tiles(ds.stream, precision, src, lv, ct, ctr, counters, persistent_blocks(L->device));
+    if (e == hipSuccess) e = launch_deposit_tiles(ds.stream, precision, src, ctr, counters, L->device);
     if (e == hipSuccess) e = hipEventRecord(ds.evk1, ds.stream);
     if (e != hipSuccess) return hip_fail(e, "batched deposit launch");
     T->last_candidates = cs.n_cand;

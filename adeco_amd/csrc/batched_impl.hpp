@@ -1,5 +1,5 @@
-// Candidate-space machinery of the batched deposit, shared by the octree leaf deposit
-// (batched.hip) and the request server (serve.hip).
+// Candidate-space machinery of the batched deposit, shared by the octree leaf deposit and the
+// request server (both in batched.hip / serve.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,14 +11,14 @@
 
 namespace adeco {
 
-constexpr int kTile = 256;  // candidates per tile == lanes per workgroup
+constexpr int kWaveTile = 64;  // candidates per tile == lanes per wavefront
 
 // One output grid: a leaf or a request ROI.  gmin/delta/scale in FP64; base = first cell of the
 // grid in the grid buffer; cells i + dims*(j + dims*k).
 struct GridDesc {
     double gmin[3];
     double delta[3];
-    double r[3];  // delta_a * scale_a
+    double r[3];  // delta_a * scale_a: the cell edge in the scaled coordinates
     double scale[3];
     double cell_vol;
     long long base;
@@ -28,11 +28,21 @@ struct GridDesc {
 
 GridDesc make_grid_desc(const GridGeom& g, long long base);
 
-// A candidate space: binned (grid, tet) pairs x covered cells, cut into tiles of kTile candidates.
+// Everything the deposit needs about one binned (grid, tet) pair, written once by the pair setup:
+// one 128-byte line per pair.
+struct alignas(128) PairData {
+    double S[12];              // scale_a * (v_q - gmin_a), vertices in the oriented order
+    double mcell;              // tet_mass * cell_vol / |V|
+    unsigned long long range;  // covered cells, 10-bit lo / extent per axis (extent 0: dead pair)
+    int grid;
+    int pad_[3];
+};
+static_assert(sizeof(PairData) == 128, "PairData is one cache line");
+
+// A candidate space: binned (grid, tet) pairs x covered cells, cut into tiles of kWaveTile
+// consecutive candidates.
 struct TileSource {
-    const int32_t* pairs;        // lattice tet id per pair
-    const int32_t* pair_grid;    // grid index per pair
-    const unsigned long long* range;  // packed covered range per pair (10-bit lo / extent per axis)
+    const PairData* pd;
     const int64_t* cand_off;     // candidate offsets, n_pairs + 1
     const int32_t* tile_first;   // first pair of each tile, n_tiles + 1
     int64_t n_tiles, n_cand;
@@ -43,9 +53,8 @@ struct TileSource {
 
 // Device work arrays of one candidate space (allocated from one slab).
 struct CandidateSpace {
-    int32_t* pair_grid = nullptr;
+    PairData* pd = nullptr;
     unsigned* cand = nullptr;
-    unsigned long long* range = nullptr;
     int64_t* cand_off = nullptr;
     int64_t* block_sum = nullptr;
     int64_t* total = nullptr;
@@ -53,27 +62,34 @@ struct CandidateSpace {
     int64_t n_pairs = 0, n_cand = 0, n_tiles = 0;
 };
 
+// Launch shape of the persistent deposit: tiles taken per work-queue pop and resident workgroups
+// (of 4 wavefronts) per CU.  Chosen by adeco_autotune_batched; defaults fit MI355X.
+struct LaunchShape {
+    int chunk = 8;
+    int blocks_per_cu = 2;
+};
+LaunchShape& launch_shape();
+
 // Bytes of the work slab for n_pairs pairs (tile map excluded: it needs n_cand).
 size_t candidate_space_bytes(int64_t n_pairs);
 size_t tile_map_bytes(int64_t n_cand);
 
 // Lays out `cs` in `slab` and runs pair setup + scan (asynchronous; n_cand is read back with a
-// synchronous copy).  pair_grid_in: nullptr -> grid = leaf from leaf_off (CSR over grids).
+// synchronous copy).  Pairs are grid-major: grid_off is the CSR offset of each grid's pairs.
 adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, const PackedCorners& ct,
-                                   const int32_t* pairs, int64_t n_pairs, const int64_t* leaf_off, int n_grids,
-                                   const GridDesc* d_grids, void* slab, CandidateSpace& cs);
+                                   const int32_t* pairs, int64_t n_pairs, const int64_t* grid_off, int n_grids,
+                                   const GridDesc* d_grids, double tet_mass, void* slab, CandidateSpace& cs);
 // Writes the tile map into `tiles` (tile_map_bytes(cs.n_cand) bytes).
 adeco_status build_tile_map(hipStream_t stream, CandidateSpace& cs, int32_t* tiles);
 
 // Persistent deposit over one candidate space; counters[0] += FP64 recomputes, counters[1] +=
-// partial cells.  tile_counter must be zero.
-hipError_t launch_deposit_tiles(hipStream_t stream, int precision, const TileSource& src, const LatticeView& lat,
-                                const PackedCorners& ct, unsigned* tile_counter, unsigned long long* counters,
-                                int grid_blocks);
+// partial cells.  queue_counter must be zero.
+hipError_t launch_deposit_tiles(hipStream_t stream, int precision, const TileSource& src, unsigned* queue_counter,
+                                unsigned long long* counters, int device);
 
 // One serving step: entries (source slot, tile) over several candidate spaces in one launch.
 hipError_t launch_deposit_step(hipStream_t stream, int precision, const TileSource* d_sources,
-                               const int2* d_entries, int n_entries, const LatticeView& lat, const PackedCorners& ct,
-                               unsigned* entry_counter, unsigned long long* counters, int grid_blocks);
+                               const int2* d_entries, int n_entries, unsigned* queue_counter,
+                               unsigned long long* counters, int device);
 
 }  // namespace adeco

@@ -64,39 +64,70 @@ def max_rel_err(a, b, floor):
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
 
 
+def _stdout_to_stderr(fn):
+    """Runs fn with file descriptor 1 pointed at stderr: the reference library logs to stdout, and
+    stdout must carry the JSON line alone."""
+    import ctypes
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        return fn()
+    finally:
+        ctypes.CDLL(None).fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
 def cpu_baseline(lat, tree_leaves_order, budget_s=20.0):
-    """Times the reference CPU path (or the oracle port) on a sample of leaves; returns a dict."""
+    """Times the reference CPU path (or the oracle port) on this host's cores; returns a dict.
+
+    The reference deposits the leaves in its processing order, one leaf per worker thread at a
+    time, each thread into its own float grid, until every leaf is done or budget_s has passed;
+    a partial run is extrapolated by the meta-patch share of the leaves it finished."""
+    import threading
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from tests import _oracle as O  # checker side only
     from tests._octree_io import ref_octree
     threads = max(1, min(16, os.cpu_count() or 1))
     ref = O.load_ref()
     if ref is not None:
-        d = ref_octree(lat, MAX_TETS, MAX_LEVEL)
-        order = d["order"]
+        d = _stdout_to_stderr(lambda: ref_octree(lat, MAX_TETS, MAX_LEVEL))
+        order = [int(x) for x in d["order"]]
         dfs = list(d["dfs"])
         n_meta = np.diff(d["mc_offsets"])
         meta_of = {int(node): int(n_meta[i]) for i, node in enumerate(dfs)}
-        sample = [int(x) for x in order[::4]]
-        grids = [np.zeros(DIMS ** 3, np.float32) for _ in range(threads)]
-
-        def one(i_node):
-            i, node = i_node
-            g = grids[i % threads]
-            assert ref.ref_octree_deposit_leaf(node, DIMS, O.P(g), None, None, None) == 0
-            return node
-
+        lock = threading.Lock()
+        todo = iter(order)
+        done = []
         t0 = time.perf_counter()
-        with concurrent.futures.ThreadPoolExecutor(threads) as ex:
-            list(ex.map(one, enumerate(sample)))
+
+        def worker():
+            g = np.zeros(DIMS ** 3, np.float32)
+            while time.perf_counter() - t0 < budget_s:
+                with lock:
+                    node = next(todo, None)
+                if node is None:
+                    return
+                g[:] = 0
+                assert ref.ref_octree_deposit_leaf(node, DIMS, O.P(g), None, None, None) == 0
+                with lock:
+                    done.append(node)
+
+        pool = [threading.Thread(target=worker) for _ in range(threads)]
+        for th in pool:
+            th.start()
+        for th in pool:
+            th.join()
         wall = time.perf_counter() - t0
-        frac = sum(meta_of[n] for n in sample) / float(sum(meta_of.values()))
+        frac = sum(meta_of[n] for n in done) / float(sum(meta_of.values()))
         est_wall = wall / frac
+        whole = len(done) == len(order)
         return {"value": lat.num_tets / est_wall, "unit": "tets/s", "cores": threads, "kind": "reference",
-                "sample": f"{len(sample)} of {len(dfs)} leaves (every 4th in the reference's processing order, "
-                          f"{frac:.3f} of the meta-patch work) deposited by the compiled reference "
-                          f"DepositWrapper::deposit_tets (REF_EFFECTIVE tets, float) on {threads} threads in "
-                          f"{wall:.2f} s; whole-job time extrapolated by meta-patch share"}
+                "sample": (f"{'all' if whole else len(done)} of {len(order)} leaves"
+                           f"{'' if whole else f' ({frac:.3f} of the meta-patch work, whole-job time extrapolated by that share)'}"
+                           f" deposited by the compiled reference DepositWrapper::deposit_tets (REF_EFFECTIVE tets, "
+                           f"float grids, leaf gather included) on {threads} threads in {wall:.2f} s")}
     # fallback: the oracle port, FP64, on a fixed tet sample of the whole lattice
     box = np.asarray(A.cubic_root_box(lat.bbox()))
     ids = np.arange(0, lat.num_tets, 97, dtype=np.int32)
@@ -192,7 +223,7 @@ def main():
                         "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                         "flops_per_pair": flops_per_pair, "pairs_per_launch": n_pairs,
-                        "kernel_ms": main_ms, "kernel": "k_deposit_batched<FP64>"}
+                        "kernel_ms": main_ms, "kernel": "k_deposit_waves<FP64>"}
         except Exception as exc:  # the oracle is absent: report the roofline as unmeasured
             log(f"[bench] roofline skipped: {exc}")
 
