@@ -318,6 +318,81 @@ class Octree:
             pass
 
 
+class Server:
+    """Continuous-batching server of region-of-interest density requests over a resident lattice.
+
+    ``submit(roi, dims)`` queues a request for the FP32 density (mass / cell volume) of every tet
+    whose AABB strictly overlaps the box ``roi`` (x0,x1,y0,y1,z0,z1) on a dims^3 grid over it;
+    ``wait(id)`` returns the density as a (dims, dims, dims) array indexed [k, j, i] with the
+    request's timing.  Requests are admitted while a slot is free and the memory budget allows,
+    and every step launch shares its tiles among all admitted requests.
+    """
+
+    def __init__(self, lattice: ResidentLattice, decomposition: int = DECOMP_KUHN, tet_mass: float = 1.0 / 6.0,
+                 precision: int = PREC_FP64, max_inflight: int = 16, step_tiles: int = 0, memory_budget: int = 0):
+        self.lattice = lattice
+        cfg = _lib.ServeConfig(int(decomposition), int(precision), float(tet_mass), int(max_inflight), 0,
+                               int(step_tiles), int(memory_budget))
+        h = _lib.c_vp()
+        _check(load().adeco_serve_create(lattice.handle, _lib.ctypes.byref(cfg), _lib.ctypes.byref(h)), "serve_create()")
+        self.handle = h
+        self._out = {}
+
+    def submit(self, roi, dims: int) -> int:
+        roi = np.ascontiguousarray(roi, np.float64)
+        out = np.empty(int(dims) ** 3, np.float32)
+        rid = _lib.ctypes.c_uint64()
+        _check(load().adeco_serve_submit(self.handle, ptr(roi), int(dims), ptr(out), _lib.ctypes.byref(rid)),
+               "serve_submit()")
+        self._out[rid.value] = out
+        return rid.value
+
+    def wait(self, request_id: int, timeout: float = -1.0):
+        info = _lib.RequestInfo()
+        info.status = -1  # left unset by a wait that timed out: the request still owns its buffer
+        st = load().adeco_serve_wait(self.handle, int(request_id), float(timeout), _lib.ctypes.byref(info))
+        out = self._out.pop(int(request_id), None) if info.status != -1 else None
+        _check(st, "serve_wait()")
+        d = int(info.dims)
+        return out.reshape(d, d, d), {"n_tets": int(info.n_tets), "n_candidates": int(info.n_candidates),
+                                      "t_submit": info.t_submit, "t_admit": info.t_admit, "t_done": info.t_done,
+                                      "latency": info.t_done - info.t_submit}
+
+    def stats(self) -> dict:
+        st = _lib.ServeStats()
+        _check(load().adeco_serve_stats_get(self.handle, _lib.ctypes.byref(st)), "serve_stats()")
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            load().adeco_serve_destroy(self.handle)
+            self.handle = _lib.c_vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+
+def roi_deposit_plain(lattice: ResidentLattice, roi, dims: int, decomposition: int = DECOMP_KUHN,
+                      tet_mass: float = 1.0 / 6.0):
+    """The serial baseline of Server: one request, plain one-tet-per-thread kernel, synchronous.
+    Returns the (dims, dims, dims) FP32 density and the number of selected tets."""
+    roi = np.ascontiguousarray(roi, np.float64)
+    out = np.empty(int(dims) ** 3, np.float32)
+    n = _lib.ctypes.c_int64()
+    _check(load().adeco_roi_deposit_plain(lattice.handle, int(decomposition), float(tet_mass), ptr(roi), int(dims),
+                                          ptr(out), _lib.ctypes.byref(n)), "roi_deposit_plain()")
+    return out.reshape(int(dims), int(dims), int(dims)), int(n.value)
+
+
 def deposit_soup(tets, bbox, dims: int, tet_mass: float = 1.0, device: int = 0, mode: int = MODE_PLAIN,
                  precision: int = PREC_FP64, stats: bool = False):
     """Deposits a float32 tet soup [n, 4, 3] onto one dims^3 FP64 grid."""

@@ -56,6 +56,13 @@ SIGNATURES = {
     "adeco_octree_leaf_tets": (ctypes.c_int64, [c_vp, ctypes.c_int64, c_vp]),
     "adeco_octree_leaf_boxes": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp]),
     "adeco_last_timings": (None, [c_vp]),
+    "adeco_serve_create": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(c_vp)]),
+    "adeco_serve_submit": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_vp, c_u64p]),
+    "adeco_serve_wait": (ctypes.c_int, [c_vp, ctypes.c_uint64, ctypes.c_double, c_vp]),
+    "adeco_serve_stats_get": (ctypes.c_int, [c_vp, c_vp]),
+    "adeco_serve_destroy": (ctypes.c_int, [c_vp]),
+    "adeco_roi_deposit_plain": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_double, c_vp, ctypes.c_int, c_vp,
+                                               c_i64p]),
     "adeco_selftest_clip_vol6_f64": (ctypes.c_double, [c_vp]),
     "adeco_selftest_clip_vol6_f32": (ctypes.c_float, [c_vp]),
     "adeco_selftest_classify_f64": (ctypes.c_int, [c_vp]),
@@ -64,6 +71,28 @@ SIGNATURES = {
     "adeco_selftest_deposit_soup_f64": (ctypes.c_int, [c_vp, ctypes.c_int64, c_vp, ctypes.c_int, ctypes.c_double,
                                                        c_vp]),
 }
+
+
+class ServeConfig(ctypes.Structure):
+    """adeco_serve_config"""
+    _fields_ = [("decomposition", ctypes.c_int), ("precision", ctypes.c_int), ("tet_mass", ctypes.c_double),
+                ("max_inflight", ctypes.c_int), ("pad_", ctypes.c_int), ("step_tiles", ctypes.c_int64),
+                ("memory_budget", ctypes.c_uint64)]
+
+
+class RequestInfo(ctypes.Structure):
+    """adeco_request_info"""
+    _fields_ = [("status", ctypes.c_int), ("dims", ctypes.c_int), ("n_tets", ctypes.c_int64),
+                ("n_candidates", ctypes.c_int64), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
+
+
+class ServeStats(ctypes.Structure):
+    """adeco_serve_stats"""
+    _fields_ = [("requests_done", ctypes.c_uint64), ("steps", ctypes.c_uint64), ("tiles", ctypes.c_int64),
+                ("tets", ctypes.c_int64), ("busy_s", ctypes.c_double), ("device_bytes", ctypes.c_uint64),
+                ("pending", ctypes.c_int64), ("active", ctypes.c_int64)]
+
 
 _lib = None
 

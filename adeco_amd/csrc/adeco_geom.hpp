@@ -129,40 +129,66 @@ ADECO_HD R vol6(const V3<R>& a, const V3<R>& b, const V3<R>& c, const V3<R>& d) 
 template <class R>
 ADECO_HD R absr(R x) { return x < R(0) ? -x : x; }
 
+// a / b for well-scaled positive denominators: the hardware reciprocal refined by two Newton steps
+// and one correction of the quotient (within an ulp or two of the IEEE quotient, at about half the
+// instructions of the IEEE division sequence).
+ADECO_HD double qdiv(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    r = fma(fma(-b, r, 1.0), r, r);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+#else
+    return a / b;
+#endif
+}
+ADECO_HD float qdiv(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float q = a * r;
+    return fmaf(fmaf(-b, q, a), r, q);
+#else
+    return a / b;
+#endif
+}
+
 // Fraction of a tet's volume on the far side (d > 0) of a plane, from the 4 vertex distances.
+// With a = the beyond magnitudes, c = the near ones (|d|), and D = the product over the vertex
+// pairs on opposite sides of (|d_i| + |d_j|) (= |d_i - d_j|, no cancellation):
+//   1 beyond (a):        a^3 / D
+//   2 beyond (a, b):     [ (ab)^2 + ab (a+b)(c+e) + ce ((a+b)^2 - ab) ] / D
+//   3 beyond (near c):   1 - c^3 / D
+// The products of the beyond and of the near magnitudes and their sums select a, b, c, e without
+// sorting, so one division and no branches serve every case.
 template <class R>
 ADECO_HD R beyond_fraction(R d0, R d1, R d2, R d3) {
-    const int n = int(d0 > R(0)) + int(d1 > R(0)) + int(d2 > R(0)) + int(d3 > R(0));
-    if (n == 0) return R(0);
-    if (n == 4) return R(1);
-    // sort so the beyond vertices come first (values only; order within groups is irrelevant)
-    R a0 = d0, a1 = d1, a2 = d2, a3 = d3;
-#define ADECO_SWAP_IF(p, q) \
-    if (!(p > R(0)) && (q > R(0))) { const R t_ = p; p = q; q = t_; }
-    ADECO_SWAP_IF(a0, a1) ADECO_SWAP_IF(a1, a2) ADECO_SWAP_IF(a2, a3)
-    ADECO_SWAP_IF(a0, a1) ADECO_SWAP_IF(a1, a2) ADECO_SWAP_IF(a0, a1)
-#undef ADECO_SWAP_IF
-    if (n == 1) {
-        const R a = a0, b1 = -a1, b2 = -a2, b3 = -a3;
-        return (a * a * a) / ((a + b1) * (a + b2) * (a + b3));
-    }
-    if (n == 2) {
-        const R a = a0, b = a1, c = -a2, e = -a3;
-        const R num = a * a * b * b + a * b * (a + b) * (c + e) + c * e * (a * a + a * b + b * b);
-        return num / (((a + c) * (a + e)) * ((b + c) * (b + e)));
-    }
-    // n == 3: the near vertex a3 <= 0 sees the other three beyond
-    const R c = -a3, p = a0, q = a1, r = a2;
-    return R(1) - (c * c * c) / ((c + p) * (c + q) * (c + r));
+    const bool b0 = d0 > R(0), b1 = d1 > R(0), b2 = d2 > R(0), b3 = d3 > R(0);
+    const int n = int(b0) + int(b1) + int(b2) + int(b3);
+    const R m0 = absr(d0), m1 = absr(d1), m2 = absr(d2), m3 = absr(d3);
+    const R one = R(1), zero = R(0);
+    const R den = ((b0 != b1) ? m0 + m1 : one) * ((b0 != b2) ? m0 + m2 : one) * ((b0 != b3) ? m0 + m3 : one) *
+                  ((b1 != b2) ? m1 + m2 : one) * ((b1 != b3) ? m1 + m3 : one) * ((b2 != b3) ? m2 + m3 : one);
+    const R p = (b0 ? m0 : one) * (b1 ? m1 : one) * (b2 ? m2 : one) * (b3 ? m3 : one);
+    const R q = (b0 ? one : m0) * (b1 ? one : m1) * (b2 ? one : m2) * (b3 ? one : m3);
+    const R sb = (b0 ? m0 : zero) + (b1 ? m1 : zero) + (b2 ? m2 : zero) + (b3 ? m3 : zero);
+    const R sn = (b0 ? zero : m0) + (b1 ? zero : m1) + (b2 ? zero : m2) + (b3 ? zero : m3);
+    const R num2 = p * p + p * sb * sn + q * (sb * sb - p);
+    const R e = n == 1 ? p : q;
+    const R num = n == 2 ? num2 : e * e * e;
+    const R f = (n == 0 || n == 4) ? zero : qdiv(num, den);
+    return n == 4 ? one : (n == 3 ? one - f : f);
 }
 
 // |vol6| of the part of tet (a,b,c,d) inside the slab 0 <= z <= 1.
 template <class R>
 ADECO_HD R zslab_vol6(const V3<R>& a, const V3<R>& b, const V3<R>& c, const V3<R>& d) {
     const R w = absr(vol6(a, b, c, d));
-    const R below = beyond_fraction(-a.z, -b.z, -c.z, -d.z);
-    const R above = beyond_fraction(a.z - R(1), b.z - R(1), c.z - R(1), d.z - R(1));
-    const R f = R(1) - below - above;
+    const R zmin = fmin(fmin(a.z, b.z), fmin(c.z, d.z));
+    const R zmax = fmax(fmax(a.z, b.z), fmax(c.z, d.z));
+    R f = R(1);
+    if (zmin < R(0)) f -= beyond_fraction(-a.z, -b.z, -c.z, -d.z);
+    if (zmax > R(1)) f -= beyond_fraction(a.z - R(1), b.z - R(1), c.z - R(1), d.z - R(1));
     return f > R(0) ? w * f : R(0);
 }
 
@@ -176,7 +202,7 @@ struct Stage {
 
 template <class R>
 ADECO_HD V3<R> cut_point(const V3<R>& in, R din, const V3<R>& out, R dout, int axis, R c) {
-    const R t = din / (din - dout);  // din > 0 >= dout: t in (0, 1]
+    const R t = qdiv(din, din - dout);  // din > 0 >= dout: t in (0, 1]
     V3<R> p = mk(in.x + t * (out.x - in.x), in.y + t * (out.y - in.y), in.z + t * (out.z - in.z));
     set_comp(p, axis, c);
     return p;
@@ -285,6 +311,19 @@ struct ClipCounters {
     uint32_t stages, subtets;
 };
 
+// z-slab volumes of the sub-tets of one stage
+template <class R>
+ADECO_HD R stage_zslab(const Stage<R>& st, ClipCounters* cnt) {
+    if (cnt) cnt->subtets += uint32_t(st.count);
+    R v = R(0);
+    for (int k = 0; k < st.count; ++k) {
+        V3<R> q[4];
+        take(st, k, q);
+        v += zslab_vol6(q[0], q[1], q[2], q[3]);
+    }
+    return v;
+}
+
 // 6 x |t ∩ [0,1]^3| (t in cell units, any orientation).
 template <class R>
 ADECO_HD R clip_unit_cube_vol6(const V3<R> t[4], ClipCounters* cnt = nullptr) {
@@ -295,50 +334,40 @@ ADECO_HD R clip_unit_cube_vol6(const V3<R> t[4], ClipCounters* cnt = nullptr) {
         if (cnt) cnt->subtets += 1;
         return zslab_vol6(t[0], t[1], t[2], t[3]);
     }
-    R vol = R(0);
     int ax;
     R c;
     bool ge;
     Stage<R> s1, s2, s3, s4;
-    V3<R> q1[4], q2[4], q3[4], q4[4];
+    V3<R> q1[4], q2[4], q3[4];
     face_of(act, 0, ax, c, ge);
     clip_stage(t, ax, c, ge, s1);
     if (cnt) cnt->stages += 1;
+    if (nact == 1) return stage_zslab(s1, cnt);
+    R vol = R(0);
     for (int k1 = 0; k1 < s1.count; ++k1) {
         take(s1, k1, q1);
-        if (nact == 1) {
-            if (cnt) cnt->subtets += 1;
-            vol += zslab_vol6(q1[0], q1[1], q1[2], q1[3]);
-            continue;
-        }
         face_of(act, 1, ax, c, ge);
         clip_stage(q1, ax, c, ge, s2);
         if (cnt) cnt->stages += 1;
+        if (nact == 2) {
+            vol += stage_zslab(s2, cnt);
+            continue;
+        }
         for (int k2 = 0; k2 < s2.count; ++k2) {
             take(s2, k2, q2);
-            if (nact == 2) {
-                if (cnt) cnt->subtets += 1;
-                vol += zslab_vol6(q2[0], q2[1], q2[2], q2[3]);
-                continue;
-            }
             face_of(act, 2, ax, c, ge);
             clip_stage(q2, ax, c, ge, s3);
             if (cnt) cnt->stages += 1;
+            if (nact == 3) {
+                vol += stage_zslab(s3, cnt);
+                continue;
+            }
             for (int k3 = 0; k3 < s3.count; ++k3) {
                 take(s3, k3, q3);
-                if (nact == 3) {
-                    if (cnt) cnt->subtets += 1;
-                    vol += zslab_vol6(q3[0], q3[1], q3[2], q3[3]);
-                    continue;
-                }
                 face_of(act, 3, ax, c, ge);
                 clip_stage(q3, ax, c, ge, s4);
                 if (cnt) cnt->stages += 1;
-                for (int k4 = 0; k4 < s4.count; ++k4) {
-                    take(s4, k4, q4);
-                    if (cnt) cnt->subtets += 1;
-                    vol += zslab_vol6(q4[0], q4[1], q4[2], q4[3]);
-                }
+                vol += stage_zslab(s4, cnt);
             }
         }
     }

@@ -486,13 +486,13 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
 }
 
 template <int kPrec>
-__global__ void __launch_bounds__(kBlock) k_deposit_waves(TileSource src, int chunk, unsigned* __restrict__ queue_counter,
+__global__ void __launch_bounds__(kBlock, 2) k_deposit_waves(TileSource src, int chunk, unsigned* __restrict__ queue_counter,
                                                           unsigned long long* __restrict__ counters) {
     wave_loop<kPrec>(OneSource{src}, nullptr, src.n_tiles, chunk, queue_counter, counters);
 }
 
 template <int kPrec>
-__global__ void __launch_bounds__(kBlock) k_deposit_step(const TileSource* __restrict__ sources,
+__global__ void __launch_bounds__(kBlock, 2) k_deposit_step(const TileSource* __restrict__ sources,
                                                          const int2* __restrict__ entries, int n_entries, int chunk,
                                                          unsigned* __restrict__ queue_counter,
                                                          unsigned long long* __restrict__ counters) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "runtime.hpp"
 
@@ -15,6 +16,9 @@ struct adeco_lattice_s {
     float* d_x = nullptr;
     float* d_y = nullptr;
     float* d_z = nullptr;
+    // float AABBs of the meta-cells (7^3 lattice cubes), [M][6], filled by the first ROI selection
+    std::vector<float> meta_box;
+    int meta_dims[3] = {0, 0, 0};
     ~adeco_lattice_s() {
         if (d_x) (void)hipFree(d_x);
         if (d_y) (void)hipFree(d_y);

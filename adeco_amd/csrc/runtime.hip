@@ -84,6 +84,12 @@ adeco_status DevBuf::reserve(size_t nbytes) {
     return ADECO_OK;
 }
 
+void DevBuf::free_all() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
 DevBuf::~DevBuf() {
     if (ptr) (void)hipFree(ptr);
 }

@@ -24,6 +24,7 @@ struct DevBuf {
     void* ptr = nullptr;
     size_t cap = 0;
     adeco_status reserve(size_t nbytes);
+    void free_all();
     template <class T> T* as() const { return static_cast<T*>(ptr); }
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;

@@ -157,6 +157,99 @@ def algorithmic_flops(lat, tree, leaves, decomp, n_sample=24):
     return flops / max(pairs, 1), int(counts.sum())
 
 
+SERVE_N = 256
+SERVE_RES = (64, 128, 256, 512)
+
+
+def serve_requests(count, seed):
+    """A request set for the Poisson streams: unit-rate exponential gaps (divided by the rate),
+    ROI boxes (side U[5,50] Mpc/h, centre U[0,100)^3) and resolutions drawn from 64^3..512^3."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0, count)
+    gaps[0] = 0.0
+    side = rng.uniform(5.0, 50.0, count)
+    centre = rng.uniform(0.0, 100.0, (count, 3))
+    dims = rng.choice(SERVE_RES, count)
+    rois = np.empty((count, 6))
+    rois[:, 0::2] = centre - side[:, None] / 2
+    rois[:, 1::2] = centre + side[:, None] / 2
+    return np.cumsum(gaps), rois, dims
+
+
+def percentiles(lat):
+    a = np.asarray(lat)
+    return float(np.percentile(a, 50)), float(np.percentile(a, 99))
+
+
+def serve_study(rates, count, seed=5, max_inflight=16):
+    """Serving: the resident 256^3 lattice (the 128^3 workload's field type, rms 3 spacings) answers
+    Poisson streams of ROI requests; p50/p99 latency and tets/s of the continuous-batching server
+    (real-time arrivals) against serial plain-GPU processing of the same requests (one at a time,
+    arrival order: FIFO over the measured service times)."""
+    import threading
+    t0 = time.perf_counter()
+    lat = make_lattice(SERVE_N, RMS, seed=seed)
+    log(f"[serve] lattice {SERVE_N}^3 generated in {time.perf_counter() - t0:.1f}s")
+    unit_arr, rois, dims = serve_requests(count, seed)
+    out = {"lattice": SERVE_N, "tets": lat.num_tets, "requests": count, "max_inflight": max_inflight, "rates": []}
+    with A.ResidentLattice.from_lattice(lat) as rl:
+        # serial plain GPU service times (independent of the arrival rate)
+        service, plain_keep, n_tets = [], {}, []
+        for i in range(count):
+            s0 = time.perf_counter()
+            g, n = A.roi_deposit_plain(rl, rois[i], int(dims[i]), A.DECOMP_KUHN, TET_MASS)
+            service.append(time.perf_counter() - s0)
+            n_tets.append(n)
+            if i < 4:
+                plain_keep[i] = g
+        tets = float(np.sum(n_tets))
+        log(f"[serve] serial plain: {np.sum(service):.2f} s for {count} requests ({tets / np.sum(service):.3e} tets/s)")
+        out["plain_serial_busy_s"] = float(np.sum(service))
+        out["plain_serial_tets_per_s_busy"] = tets / float(np.sum(service))
+        for rate in rates:
+            arr = unit_arr / rate
+            lat_s, err = [], 0.0
+            with A.Server(rl, A.DECOMP_KUHN, TET_MASS, max_inflight=max_inflight) as srv:
+                ids = [None] * count
+                start = time.perf_counter()
+
+                def submitter():
+                    for i in range(count):
+                        dt = arr[i] - (time.perf_counter() - start)
+                        if dt > 0:
+                            time.sleep(dt)
+                        ids[i] = srv.submit(rois[i], int(dims[i]))
+
+                th = threading.Thread(target=submitter)
+                th.start()
+                th.join()
+                for i in range(count):
+                    g, info = srv.wait(ids[i])
+                    lat_s.append(info["latency"])
+                    if i in plain_keep:
+                        cell_vol = np.prod((rois[i][1::2] - rois[i][0::2]) / dims[i])
+                        floor = 1e-8 * TET_MASS / cell_vol
+                        want = plain_keep[i].astype(np.float64)
+                        err = max(err, float(np.max(np.abs(g.astype(np.float64) - want) / np.maximum(np.abs(want), floor))))
+                makespan = time.perf_counter() - start
+                st = srv.stats()
+            p50, p99 = percentiles(lat_s)
+            finish, lat_p = 0.0, []
+            for i in range(count):
+                finish = max(finish, arr[i]) + service[i]
+                lat_p.append(finish - arr[i])
+            q50, q99 = percentiles(lat_p)
+            rec = {"rate_req_s": rate, "batched_p50_s": p50, "batched_p99_s": p99,
+                   "batched_tets_per_s": tets / makespan, "batched_makespan_s": makespan, "steps": int(st["steps"]),
+                   "plain_serial_p50_s": q50, "plain_serial_p99_s": q99, "plain_serial_tets_per_s": tets / finish,
+                   "plain_serial_makespan_s": finish, "batched_vs_plain_max_rel_err": err}
+            log(f"[serve] {rate:g} req/s: batched p50 {p50 * 1e3:.1f} ms p99 {p99 * 1e3:.1f} ms "
+                f"({tets / makespan:.3e} tets/s); serial plain p50 {q50 * 1e3:.1f} ms p99 {q99 * 1e3:.1f} ms "
+                f"({tets / finish:.3e} tets/s); max rel diff {err:.2e}")
+            out["rates"].append(rec)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,10 +257,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the plain and mixed comparison runs")
+    ap.add_argument("--serve", action="store_true", help="run only the ROI serving study (256^3 lattice)")
+    ap.add_argument("--serve-rates", default="20,50,100,200")
+    ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: every box has one MI355X; run with --gpus 1"}))
         sys.exit(2)
+    if args.serve:
+        res = serve_study([float(r) for r in args.serve_rates.split(",")], args.serve_requests)
+        print(json.dumps({"metric": "p50/p99 ROI latency (s) and tets/s, continuous batching vs serial plain GPU",
+                          "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+                          "data": "synthetic (256^3 lattice, k^-2 field at rms 3 spacings; Poisson ROI requests)",
+                          "config": {"workload": "ROI serving, side U[5,50] Mpc/h, resolutions 64^3..512^3, fp32 density"},
+                          **res}))
+        return
 
     t0 = time.perf_counter()
     lat = make_lattice(N, RMS, seed=SEED)

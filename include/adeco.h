@@ -153,6 +153,54 @@ int64_t adeco_octree_leaf_tets(adeco_octree tree, int64_t leaf, int32_t* out);
  * candidates, partial cells clipped, FP64 recomputes of the mixed-precision clip}. */
 void adeco_last_timings(double* out5);
 
+/* ---- region-of-interest density requests over the resident lattice ----
+ * The reference has no request server: these entry points apply its single-rank leaf deposit
+ * (AdDynamicLoadBalancing.h:983-1323, deposit_tets per leaf) to caller-given boxes.  A request's
+ * result is the FP32 density (mass / cell volume, as AdOctreeGridWriter.h:67-144) of every lattice
+ * tet whose float AABB strictly overlaps the box, on a dims^3 grid over the box (dims <= 1023). */
+typedef struct adeco_server_s* adeco_server;
+typedef struct {
+    int decomposition;    /* ADECO_DECOMP_* */
+    int precision;        /* ADECO_PREC_* */
+    double tet_mass;      /* mass of one tet (> 0) */
+    int max_inflight;     /* requests resident on the device at once, 1..64 */
+    int pad_;
+    int64_t step_tiles;   /* tiles of 64 (tet, cell) candidates per step launch (0: 131072) */
+    uint64_t memory_budget; /* device bytes for request buffers (0: no limit but the device's) */
+} adeco_serve_config;
+typedef struct {
+    adeco_status status;
+    int dims;
+    int64_t n_tets;       /* tets selected for the box */
+    int64_t n_candidates; /* (tet, cell) candidates of the batched deposit */
+    double t_submit, t_admit, t_done; /* seconds on the server's steady clock */
+} adeco_request_info;
+typedef struct {
+    uint64_t requests_done;
+    uint64_t steps;       /* step launches */
+    int64_t tiles;        /* tiles deposited */
+    int64_t tets;         /* tets of the completed requests */
+    double busy_s;        /* time the worker spent in steps */
+    uint64_t device_bytes; /* request buffers held */
+    int64_t pending, active;
+} adeco_serve_stats;
+/* Starts a server (a worker and a copier thread) on the lattice's device.  Requests are admitted
+ * in arrival order while a slot is free and the memory budget allows; each step launch deposits an
+ * equal share of its tiles from every admitted request (continuous batching). */
+adeco_status adeco_serve_create(adeco_lattice lat, const adeco_serve_config* cfg, adeco_server* out);
+/* Queues a request; density_out (dims^3 floats, i fastest) must stay valid until it completes. */
+adeco_status adeco_serve_submit(adeco_server srv, const double* roi6, int dims, float* density_out,
+                                uint64_t* request_id);
+/* Blocks until the request completes (timeout_s < 0: no limit) and releases its record. */
+adeco_status adeco_serve_wait(adeco_server srv, uint64_t request_id, double timeout_s, adeco_request_info* info);
+adeco_status adeco_serve_stats_get(adeco_server srv, adeco_serve_stats* out);
+/* Finishes every queued request, then stops the threads and frees the device buffers. */
+adeco_status adeco_serve_destroy(adeco_server srv);
+/* The serial baseline: the same selection and output for one box, deposited by the plain
+ * one-tet-per-thread kernel, synchronously.  n_tets_out (nullable) receives the selected tets. */
+adeco_status adeco_roi_deposit_plain(adeco_lattice lat, int decomposition, double tet_mass, const double* roi6,
+                                     int dims, float* density_out, int64_t* n_tets_out);
+
 /* ---- self-test hooks: the shared host/device geometry evaluated on the host ----
  * These exist so the CPU test suite can check the clip and classification code that the kernels
  * compile; no deposit entry point ever calls them. */

@@ -51,6 +51,20 @@ def test_no_gpu_means_edevice_not_fallback():
     assert ei.value.status == A.ADECO_EDEVICE
 
 
+def test_server_argument_validation():
+    lib = A.load()
+    h = A._lib.c_vp()
+    cfg = A._lib.ServeConfig(A.DECOMP_KUHN, A.PREC_FP64, 1.0 / 6, 4, 0, 0, 0)
+    assert lib.adeco_serve_create(None, A._lib.ctypes.byref(cfg), A._lib.ctypes.byref(h)) == A.ADECO_EINVAL
+    assert lib.adeco_serve_submit(None, None, 8, None, None) == A.ADECO_EINVAL
+    assert lib.adeco_serve_wait(None, 1, 0.0, None) == A.ADECO_EINVAL
+    assert lib.adeco_serve_stats_get(None, None) == A.ADECO_EINVAL
+    assert lib.adeco_serve_destroy(None) == A.ADECO_OK
+    roi = np.array([0, 1, 0, 1, 0, 1], np.float64)
+    out = np.zeros(8, np.float32)
+    assert lib.adeco_roi_deposit_plain(None, 2, 1.0, A.ptr(roi), 2, A.ptr(out), None) == A.ADECO_EINVAL
+
+
 def test_boundary_argument_validation_precedes_device_use():
     lat = make_lattice(4, 0.5)
     pat = meta_patches(lat)
@@ -123,6 +137,12 @@ def test_beyond_fraction_closed_forms():
         s = (t - np.array([lo[0], lo[1], 0.0])) / ext
         w = o.oracle_clip_unit_cube_f64(O.P(np.ascontiguousarray(s)), None) * ext ** 3
         assert got == pytest.approx(w / (abs(_vol6(t)) / 6), rel=1e-10, abs=1e-12)
+    # vertices exactly on the plane count as near; every case of 0-4 beyond
+    for d, want in (([1, 0, 0, 0], 1.0), ([1, 1, 1, 0], 1.0), ([1, 1, 0, 0], 1.0), ([-1, 0, 0, 0], 0.0),
+                    ([0, 0, 0, 0], 0.0), ([2, 2, 2, 2], 1.0), ([-1, -1, -1, -1], 0.0),
+                    ([1, -1, -1, -1], 1 / 8), ([1, 1, 1, -1], 7 / 8), ([1, 1, -1, -1], 0.5)):
+        got = lib.adeco_selftest_beyond_fraction_f64(A.ptr(np.ascontiguousarray(d, np.float64)))
+        assert got == pytest.approx(want, rel=1e-15, abs=1e-16), d
 
 
 def test_kernel_classification_agrees_with_clip():
