@@ -286,6 +286,17 @@ class Octree:
         lib.adeco_octree_leaf_tets(self.handle, int(leaf), ptr(out))
         return out[:n]
 
+    def autotune(self, dims: int, decomposition: int = DECOMP_REF_EFFECTIVE, tet_mass: float = 1.0,
+                 precision: int = PREC_FP64, reps: int = 2) -> dict:
+        """Picks the batched kernel's launch shape (tiles per work-queue pop, workgroups per CU) by
+        timing this octree's leaf deposit; the choice stays in effect for later deposits."""
+        c, b, ms = _lib.ctypes.c_int(), _lib.ctypes.c_int(), _lib.ctypes.c_double()
+        _check(load().adeco_autotune_batched(self.handle, int(dims), int(decomposition), float(tet_mass), int(precision),
+                                             int(reps), _lib.ctypes.byref(c), _lib.ctypes.byref(b), _lib.ctypes.byref(ms)),
+               "autotune_batched()")
+        self.dims = int(dims)
+        return {"chunk": c.value, "blocks_per_cu": b.value, "kernel_ms": ms.value}
+
     @staticmethod
     def last_timings() -> dict:
         out = np.zeros(5, np.float64)
@@ -379,6 +390,17 @@ class Server:
             self.close()
         except Exception:  # interpreter shutdown
             pass
+
+
+def set_launch_shape(chunk: int, blocks_per_cu: int):
+    """Batched kernel launch shape: tiles of 64 candidates per work-queue pop, workgroups per CU."""
+    _check(load().adeco_set_launch_shape(int(chunk), int(blocks_per_cu)), "set_launch_shape()")
+
+
+def get_launch_shape() -> tuple:
+    c, b = _lib.ctypes.c_int(), _lib.ctypes.c_int()
+    load().adeco_get_launch_shape(_lib.ctypes.byref(c), _lib.ctypes.byref(b))
+    return c.value, b.value
 
 
 def roi_deposit_plain(lattice: ResidentLattice, roi, dims: int, decomposition: int = DECOMP_KUHN,

@@ -56,6 +56,10 @@ SIGNATURES = {
     "adeco_octree_leaf_tets": (ctypes.c_int64, [c_vp, ctypes.c_int64, c_vp]),
     "adeco_octree_leaf_boxes": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp]),
     "adeco_last_timings": (None, [c_vp]),
+    "adeco_set_launch_shape": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "adeco_get_launch_shape": (ctypes.c_int, [c_vp, c_vp]),
+    "adeco_autotune_batched": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, c_vp, c_vp, c_vp]),
     "adeco_serve_create": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(c_vp)]),
     "adeco_serve_submit": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_vp, c_u64p]),
     "adeco_serve_wait": (ctypes.c_int, [c_vp, ctypes.c_uint64, ctypes.c_double, c_vp]),

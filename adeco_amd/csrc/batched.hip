@@ -254,28 +254,46 @@ __device__ __forceinline__ int classify_margin(const V3<double> s[4], const doub
 }
 
 // Clip work class of a partial cell: the axis with the most faces cutting the tet is integrated in
-// closed form (ties -> z); the class counts the remaining x/y faces that cut and whether the
-// first cut leaves one sub-tet or a prism.
+// closed form (ties -> z); the class counts the remaining x/y faces that cut (from the cell-local
+// AABB) and whether the first of them leaves one sub-tet or a prism.  The class only groups cells
+// of similar clip work; the clip itself decides every face again.
 __device__ __forceinline__ int work_class(const V3<double> s[4], const double mn[3], const double mx[3], int& axis) {
-    int act[3];
-    for (int a = 0; a < 3; ++a) act[a] = int(mn[a] < 0.0) + int(mx[a] > 1.0);
-    axis = 2;
-    if (act[0] > act[axis]) axis = 0;
-    if (act[1] > act[axis]) axis = 1;
-    V3<double> u[4] = {s[0], s[1], s[2], s[3]};
-    rotate_axes(u, axis);
-    const int faces = active_xy_faces(u);
-    if (faces <= 0) return 0;
-    const int nact = (faces & 1) + ((faces >> 1) & 1) + ((faces >> 2) & 1) + ((faces >> 3) & 1);
-    const int f = __builtin_ctz(unsigned(faces));
-    const int ax = f >> 1;
-    const bool ge = (f & 1) == 0;
+    const bool lx = mn[0] < 0.0, hx = mx[0] > 1.0, ly = mn[1] < 0.0, hy = mx[1] > 1.0, lz = mn[2] < 0.0, hz = mx[2] > 1.0;
+    const int ax_ = int(lx) + int(hx), ay = int(ly) + int(hy), az = int(lz) + int(hz);
+    // closed-form axis: most cutting faces, ties -> z, then x
+    axis = (ax_ > az && ax_ >= ay) ? 0 : ((ay > az && ay > ax_) ? 1 : 2);
+    // the clip's x and y after rotation: (y, z) for axis 0, (z, x) for axis 1, (x, y) for axis 2
+    const bool lo0 = axis == 0 ? ly : (axis == 1 ? lz : lx), hi0 = axis == 0 ? hy : (axis == 1 ? hz : hx);
+    const bool lo1 = axis == 0 ? lz : (axis == 1 ? lx : ly), hi1 = axis == 0 ? hz : (axis == 1 ? hx : hy);
+    const int nact = int(lo0) + int(hi0) + int(lo1) + int(hi1);
+    if (nact == 0) return 0;
+    // first cutting face in the clip's order x >= 0, x <= 1, y >= 0, y <= 1
+    const bool on0 = lo0 || hi0;
+    const int fax = on0 ? (axis == 0 ? 1 : (axis == 1 ? 2 : 0)) : (axis == 0 ? 2 : (axis == 1 ? 0 : 1));
+    const bool ge = on0 ? lo0 : lo1;
+    // select comparison results, not coordinates: a select of coordinates becomes a select of
+    // addresses into a stack copy of the tet
     int nin = 0;
     for (int q = 0; q < 4; ++q) {
-        const double x = comp(u[q], ax);
-        nin += int(ge ? x > 0.0 : 1.0 - x > 0.0);
+        const bool ix = ge ? s[q].x > 0.0 : s[q].x < 1.0;
+        const bool iy = ge ? s[q].y > 0.0 : s[q].y < 1.0;
+        const bool iz = ge ? s[q].z > 0.0 : s[q].z < 1.0;
+        nin += int(fax == 0 ? ix : (fax == 1 ? iy : iz));
     }
     return 2 * nact - 1 + int(nin > 1);
+}
+
+// n / d for n < 2^30 and 1 <= d < 2^30 without the integer division sequence: the FP64 reciprocal
+// refined once is accurate to ~2^-50, so the truncated product is the quotient or one less (exact
+// multiples), fixed by one integer correction each way
+__device__ __forceinline__ unsigned udiv_small(unsigned n, unsigned d) {
+    const double dd = double(d);
+    double r = __builtin_amdgcn_rcp(dd);
+    r = fma(fma(-dd, r, 1.0), r, r);
+    unsigned q = unsigned(double(n) * r);
+    q -= unsigned(q * d > n);
+    q += unsigned((q + 1) * d <= n);
+    return q;
 }
 
 __device__ __forceinline__ unsigned lane_mask_lt(unsigned long long m) {
@@ -323,20 +341,21 @@ __device__ __forceinline__ void classify_tile(const TileSource& src, int slot, i
         if (ex) {
             const unsigned ey = unsigned(rg >> 40) & 1023u;
             const unsigned r = unsigned(c - off);
-            const int i = int(rg & 1023u) + int(r % ex);
-            const int j = int((rg >> 10) & 1023u) + int((r / ex) % ey);
-            const int k = int((rg >> 20) & 1023u) + int(r / (ex * ey));
+            const unsigned rk = udiv_small(r, ex * ey);
+            const unsigned rij = r - rk * ex * ey;
+            const unsigned rj = udiv_small(rij, ex);
+            const int i = int(rg & 1023u) + int(rij - rj * ex);
+            const int j = int((rg >> 10) & 1023u) + int(rj);
+            const int k = int((rg >> 20) & 1023u) + int(rk);
             const GridDesc& g = src.grids[pd.grid];
             V3<double> s[4];
             cell_local(pd, g, i, j, k, s);
-            double mn[3] = {s[0].x, s[0].y, s[0].z}, mx[3] = {s[0].x, s[0].y, s[0].z};
-            for (int q = 1; q < 4; ++q) {
-                const double cc[3] = {s[q].x, s[q].y, s[q].z};
-                for (int a = 0; a < 3; ++a) {
-                    mn[a] = cc[a] < mn[a] ? cc[a] : mn[a];
-                    mx[a] = cc[a] > mx[a] ? cc[a] : mx[a];
-                }
-            }
+            const double mn[3] = {fmin(fmin(s[0].x, s[1].x), fmin(s[2].x, s[3].x)),
+                                  fmin(fmin(s[0].y, s[1].y), fmin(s[2].y, s[3].y)),
+                                  fmin(fmin(s[0].z, s[1].z), fmin(s[2].z, s[3].z))};
+            const double mx[3] = {fmax(fmax(s[0].x, s[1].x), fmax(s[2].x, s[3].x)),
+                                  fmax(fmax(s[0].y, s[1].y), fmax(s[2].y, s[3].y)),
+                                  fmax(fmax(s[0].z, s[1].z), fmax(s[2].z, s[3].z))};
             // strict AABB cull against the cell box [0, r]; the reference then classifies and
             // clips against the unit cube in scaled coordinates (AdDepositTets2Grid.h:587-634)
             const bool hit = mn[0] < g.r[0] && 0.0 < mx[0] && mn[1] < g.r[1] && 0.0 < mx[1] && mn[2] < g.r[2] &&
@@ -358,22 +377,13 @@ __device__ __forceinline__ void classify_tile(const TileSource& src, int slot, i
         }
     }
     // append the partial cells to their queues (every queue holds < 64 here, so < 128 after)
-    unsigned long long any = __ballot(key >= 0);
-    while (any) {
-        const int b = __builtin_amdgcn_readfirstlane(__shfl(key, __builtin_ctzll(any), 64));
-        const unsigned long long m = __ballot(key == b);
-        const int base = L.qn[b];
-        if (key == b) {
-            const int pos = base + int(lane_mask_lt(m));
-            L.q[b][pos] = make_int2(p, cell);
-            L.qs[b][pos] = (unsigned char)slot;
-        }
-        wave_sync();
-        if (lane == 0) L.qn[b] = base + __popcll(m);
-        wave_sync();
-        n_partial += __popcll(m);
-        any &= ~m;
+    if (key >= 0) {
+        const int pos = atomicAdd(&L.qn[key], 1);
+        L.q[key][pos] = make_int2(p, cell);
+        L.qs[key][pos] = (unsigned char)slot;
     }
+    n_partial += __popcll(__ballot(key >= 0));
+    wave_sync();
 }
 
 template <int kPrec, class Sources>

@@ -63,10 +63,12 @@ struct CandidateSpace {
 };
 
 // Launch shape of the persistent deposit: tiles taken per work-queue pop and resident workgroups
-// (of 4 wavefronts) per CU.  Chosen by adeco_autotune_batched; defaults fit MI355X.
+// (of 4 wavefronts) per CU.  Chosen by adeco_autotune_batched.  Defaults from MI355X: one counter
+// pop costs ~12 ns of serialized atomic time, so a pop takes 16 tiles (1024 candidates); 4
+// workgroups per CU fill the FP32 kernel's occupancy (the FP64 kernel holds 2).
 struct LaunchShape {
-    int chunk = 8;
-    int blocks_per_cu = 2;
+    int chunk = 16;
+    int blocks_per_cu = 4;
 };
 LaunchShape& launch_shape();
 

@@ -22,6 +22,7 @@
 #include "../../include/adeco.h"
 #include "kernels.hpp"
 #include "lattice_impl.hpp"
+#include "batched_impl.hpp"
 #include "octree_impl.hpp"
 #include "runtime.hpp"
 
@@ -371,6 +372,56 @@ adeco_status adeco_deposit_leaves(adeco_octree T, int dims, int decomposition, d
         g_timings.partials = double(cnt[1]);
         g_timings.candidates = double(T->last_candidates);
     }
+    return ADECO_OK;
+}
+
+adeco_status adeco_set_launch_shape(int chunk, int blocks_per_cu) {
+    if (chunk < 1 || chunk > 1024 || blocks_per_cu < 1 || blocks_per_cu > 16)
+        return fail(ADECO_EINVAL, "set_launch_shape(): chunk must be in 1..1024 and blocks_per_cu in 1..16");
+    launch_shape().chunk = chunk;
+    launch_shape().blocks_per_cu = blocks_per_cu;
+    return ADECO_OK;
+}
+
+adeco_status adeco_get_launch_shape(int* chunk, int* blocks_per_cu) {
+    if (chunk) *chunk = launch_shape().chunk;
+    if (blocks_per_cu) *blocks_per_cu = launch_shape().blocks_per_cu;
+    return ADECO_OK;
+}
+
+// Times the batched leaf deposit over every (tiles per work-queue pop, resident workgroups per CU)
+// of a small grid, keeps the fastest (minimum kernel time over `reps` runs) as the library's launch
+// shape and reports it.  The leaf grids hold the result of the last run.
+adeco_status adeco_autotune_batched(adeco_octree T, int dims, int decomposition, double tet_mass, int precision, int reps,
+                                    int* best_chunk, int* best_blocks_per_cu, double* best_ms) {
+    if (!T || reps < 1) return fail(ADECO_EINVAL, "autotune_batched(): null handle or reps < 1");
+    static const int chunks[] = {1, 2, 4, 8, 16, 32};
+    static const int blocks[] = {1, 2, 3, 4};
+    const LaunchShape saved = launch_shape();
+    double best = 1e300;
+    LaunchShape pick = saved;
+    for (int c : chunks)
+        for (int b : blocks) {
+            launch_shape().chunk = c;
+            launch_shape().blocks_per_cu = b;
+            double t = 1e300;
+            for (int r = 0; r < reps; ++r) {
+                const adeco_status st = adeco_deposit_leaves(T, dims, decomposition, tet_mass, ADECO_MODE_BATCHED, precision);
+                if (st) {
+                    launch_shape() = saved;
+                    return st;
+                }
+                t = std::min(t, g_timings.main_ms);
+            }
+            if (t < best) {
+                best = t;
+                pick = launch_shape();
+            }
+        }
+    launch_shape() = pick;
+    if (best_chunk) *best_chunk = pick.chunk;
+    if (best_blocks_per_cu) *best_blocks_per_cu = pick.blocks_per_cu;
+    if (best_ms) *best_ms = best;
     return ADECO_OK;
 }
 

@@ -149,6 +149,16 @@ adeco_status adeco_octree_leaf_boxes(adeco_octree tree, int exact, double* out);
 /* Binned lattice tet ids of one leaf (DFS index) after adeco_octree_bin / adeco_deposit_leaves;
  * returns their number (-1 on error); out (nullable) receives them. */
 int64_t adeco_octree_leaf_tets(adeco_octree tree, int64_t leaf, int32_t* out);
+/* Launch shape of the batched kernel: tiles of 64 (tet, cell) candidates taken per work-queue pop
+ * (1..1024) and resident workgroups of 4 wavefronts per CU (1..16).  Library-wide; set it before
+ * concurrent use.  No reference counterpart: the reference's GPU deposit uses a fixed 512 x 64
+ * launch (AdDeposit2Grid.h:72-80). */
+adeco_status adeco_set_launch_shape(int chunk, int blocks_per_cu);
+adeco_status adeco_get_launch_shape(int* chunk, int* blocks_per_cu);
+/* Times the batched leaf deposit over chunk in {1,2,4,8,16,32} x blocks_per_cu in {1,2,3,4}
+ * (minimum kernel time of reps runs each), keeps the fastest as the launch shape and returns it. */
+adeco_status adeco_autotune_batched(adeco_octree tree, int dims, int decomposition, double tet_mass, int precision,
+                                    int reps, int* best_chunk, int* best_blocks_per_cu, double* best_ms);
 /* The calling thread's last leaf deposit: out5 = {total ms, dominant-kernel ms, (tet, cell)
  * candidates, partial cells clipped, FP64 recomputes of the mixed-precision clip}. */
 void adeco_last_timings(double* out5);
