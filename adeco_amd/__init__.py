@@ -329,6 +329,48 @@ class Octree:
             pass
 
 
+class DepositGraph:
+    """The batched leaf deposit of an octree captured as a hipGraph (grid clear, work-queue reset,
+    persistent deposit launch) and replayed by ``launch()``; ``leaf_grids()`` reads the graph's own
+    FP64 leaf grids (DFS leaf order)."""
+
+    def __init__(self, tree: "Octree", dims: int, decomposition: int = DECOMP_REF_EFFECTIVE, tet_mass: float = 1.0,
+                 precision: int = PREC_FP64):
+        self.tree = tree
+        self.dims = int(dims)
+        h = _lib.c_vp()
+        _check(load().adeco_deposit_graph_create(tree.handle, int(dims), int(decomposition), float(tet_mass),
+                                                 int(precision), _lib.ctypes.byref(h)), "deposit_graph_create()")
+        self.handle = h
+
+    def launch(self) -> float:
+        _check(load().adeco_deposit_graph_launch(self.handle), "deposit_graph_launch()")
+        return last_kernel_ms()
+
+    def leaf_grids(self, first: int = 0, count=None) -> np.ndarray:
+        n = self.tree.num_leaves - first if count is None else int(count)
+        out = np.empty((n, self.dims ** 3), np.float64)
+        _check(load().adeco_deposit_graph_grids(self.handle, int(first), n, ptr(out)), "deposit_graph_grids()")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            load().adeco_deposit_graph_destroy(self.handle)
+            self.handle = _lib.c_vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+
 class Server:
     """Continuous-batching server of region-of-interest density requests over a resident lattice.
 

@@ -60,6 +60,11 @@ SIGNATURES = {
     "adeco_get_launch_shape": (ctypes.c_int, [c_vp, c_vp]),
     "adeco_autotune_batched": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                               ctypes.c_int, c_vp, c_vp, c_vp]),
+    "adeco_deposit_graph_create": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                  ctypes.POINTER(c_vp)]),
+    "adeco_deposit_graph_launch": (ctypes.c_int, [c_vp]),
+    "adeco_deposit_graph_grids": (ctypes.c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_vp]),
+    "adeco_deposit_graph_destroy": (ctypes.c_int, [c_vp]),
     "adeco_serve_create": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(c_vp)]),
     "adeco_serve_submit": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_vp, c_u64p]),
     "adeco_serve_wait": (ctypes.c_int, [c_vp, ctypes.c_uint64, ctypes.c_double, c_vp]),

@@ -509,6 +509,11 @@ __global__ void __launch_bounds__(kBlock, 2) k_deposit_step(const TileSource* __
     wave_loop<kPrec>(ManySources{sources}, entries, n_entries, chunk, queue_counter, counters);
 }
 
+__global__ void k_zero_words(unsigned* p, int n) {
+    const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 int resident_blocks(int device) {
@@ -603,13 +608,16 @@ hipError_t launch_deposit_step(hipStream_t stream, int precision, const TileSour
     return hipGetLastError();
 }
 
-// Leaf deposit of an octree: one candidate space over all leaves, one persistent launch.
-adeco_status deposit_leaves_batched(adeco_octree_s* T, int dims, int decomposition, double tet_mass, int precision) {
+// Candidate space of an octree's leaf deposit in caller-owned buffers: leaf geometry (geom),
+// pair data and scan (work), tile map (tiles); grids are the leaf grids (dims^3 FP64 per leaf, DFS
+// leaf order).  Pair setup, scan and tile map run on `stream`; n_cand is read back.
+adeco_status prepare_leaves_batched(adeco_octree_s* T, hipStream_t stream, int dims, int decomposition, double tet_mass,
+                                    DevBuf& geom, DevBuf& work, DevBuf& tiles, double* grids, TileSource& src) {
     adeco_lattice_s* L = T->L;
-    DeviceState& ds = *L->ds;
     if (dims > 1023) return fail(ADECO_EINVAL, "batched deposit: dims must be < 1024");
     const int nl = int(T->leaves_dfs.size());
     const int64_t P = T->leaf_tet_offset[nl];
+    src = TileSource{nullptr, nullptr, nullptr, 0, 0, nullptr, grids, tet_mass};
     if (P == 0) return ADECO_OK;
     if (P >= (int64_t(1) << 31) - 2) return fail(ADECO_EOVERFLOW, "batched deposit: too many (leaf, tet) pairs");
     const int64_t cells = int64_t(dims) * dims * dims;
@@ -617,32 +625,49 @@ adeco_status deposit_leaves_batched(adeco_octree_s* T, int dims, int decompositi
     const int scaling = cell_scaling();
     for (int l = 0; l < nl; ++l) gd[l] = make_grid_desc(T->leaf_geom(l, dims, scaling), (long long)(l * cells));
     adeco_status st;
-    if ((st = T->d_work.reserve(candidate_space_bytes(P) + 256)) || (st = T->d_leaf_geom.reserve(gd.size() * sizeof(GridDesc))) ||
-        (st = T->d_counters.reserve(64)))
-        return st;
-    hipError_t e = hipMemcpyAsync(T->d_leaf_geom.ptr, gd.data(), gd.size() * sizeof(GridDesc), hipMemcpyHostToDevice, ds.stream);
+    if ((st = work.reserve(candidate_space_bytes(P) + 256)) || (st = geom.reserve(gd.size() * sizeof(GridDesc)))) return st;
+    hipError_t e = hipMemcpyAsync(geom.ptr, gd.data(), gd.size() * sizeof(GridDesc), hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return hip_fail(e, "leaf geometry upload");
     const LatticeView lv{L->d_x, L->d_y, L->d_z, {L->np[0], L->np[1], L->np[2]}};
     const PackedCorners ct = pack_corners(corner_table(decomposition));
     CandidateSpace cs;
-    if ((st = build_candidate_space(ds.stream, lv, ct, T->d_pairs.as<int32_t>(), P, T->d_leaf_offset.as<int64_t>(), nl,
-                                    T->d_leaf_geom.as<GridDesc>(), tet_mass, T->d_work.ptr, cs)))
+    // (synchronises the stream, so gd may go out of scope afterwards)
+    if ((st = build_candidate_space(stream, lv, ct, T->d_pairs.as<int32_t>(), P, T->d_leaf_offset.as<int64_t>(), nl,
+                                    geom.as<GridDesc>(), tet_mass, work.ptr, cs)))
         return st;
-    if ((st = T->d_tiles.reserve(tile_map_bytes(cs.n_cand)))) return st;
-    if ((st = build_tile_map(ds.stream, cs, T->d_tiles.as<int32_t>()))) return st;
+    if ((st = tiles.reserve(tile_map_bytes(cs.n_cand)))) return st;
+    if ((st = build_tile_map(stream, cs, tiles.as<int32_t>()))) return st;
+    src = TileSource{cs.pd, cs.cand_off, cs.tile_first, cs.n_tiles, cs.n_cand, geom.as<GridDesc>(), grids, tet_mass};
+    return ADECO_OK;
+}
+
+// Leaf deposit of an octree: one candidate space over all leaves, one persistent launch.
+adeco_status deposit_leaves_batched(adeco_octree_s* T, int dims, int decomposition, double tet_mass, int precision) {
+    adeco_lattice_s* L = T->L;
+    DeviceState& ds = *L->ds;
+    TileSource src;
+    adeco_status st = prepare_leaves_batched(T, ds.stream, dims, decomposition, tet_mass, T->d_leaf_geom, T->d_work,
+                                             T->d_tiles, T->d_grids.as<double>(), src);
+    if (st) return st;
+    if (src.n_tiles == 0) return ADECO_OK;
+    if ((st = T->d_counters.reserve(64))) return st;
     unsigned* ctr = T->d_counters.as<unsigned>();
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(T->d_counters.as<char>() + 16);
-    e = hipMemsetAsync(T->d_counters.ptr, 0, 64, ds.stream);
+    hipError_t e = hipMemsetAsync(T->d_counters.ptr, 0, 64, ds.stream);
     if (e != hipSuccess) return hip_fail(e, "batched counters");
-    const TileSource src{cs.pd, cs.cand_off, cs.tile_first, cs.n_tiles, cs.n_cand, T->d_leaf_geom.as<GridDesc>(),
-                         T->d_grids.as<double>(), tet_mass};
     e = hipEventRecord(ds.evk0, ds.stream);
     if (e == hipSuccess) e = launch_deposit_tiles(ds.stream, precision, src, ctr, counters, L->device);
     if (e == hipSuccess) e = hipEventRecord(ds.evk1, ds.stream);
     if (e != hipSuccess) return hip_fail(e, "batched deposit launch");
-    T->last_candidates = cs.n_cand;
+    T->last_candidates = src.n_cand;
     T->d_fallback = counters;
     return ADECO_OK;
+}
+
+// Clears n 32-bit words with a kernel (a memset node in a captured graph is avoided).
+hipError_t launch_zero_u32(unsigned* p, int n, hipStream_t stream) {
+    hipLaunchKernelGGL(k_zero_words, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
 }
 
 }  // namespace adeco

@@ -9,6 +9,8 @@
 #include "kernels.hpp"
 #include "runtime.hpp"
 
+struct adeco_octree_s;
+
 namespace adeco {
 
 constexpr int kWaveTile = 64;  // candidates per tile == lanes per wavefront
@@ -93,5 +95,13 @@ hipError_t launch_deposit_tiles(hipStream_t stream, int precision, const TileSou
 hipError_t launch_deposit_step(hipStream_t stream, int precision, const TileSource* d_sources,
                                const int2* d_entries, int n_entries, unsigned* queue_counter,
                                unsigned long long* counters, int device);
+
+// Candidate space of an octree's leaf deposit in caller-owned buffers (leaf geometry, pair data +
+// scan, tile map) over the leaf grids `grids`; src describes it for launch_deposit_tiles.
+adeco_status prepare_leaves_batched(adeco_octree_s* T, hipStream_t stream, int dims, int decomposition, double tet_mass,
+                                    DevBuf& geom, DevBuf& work, DevBuf& tiles, double* grids, TileSource& src);
+
+// Clears n 32-bit words with a kernel (a memset node in a captured graph is avoided).
+hipError_t launch_zero_u32(unsigned* p, int n, hipStream_t stream);
 
 }  // namespace adeco

@@ -104,6 +104,11 @@ ADECO_HD bool is_sentinel(float x, float y, float z) {  // AdDepositTets2Grid.h:
 
 // The 4 vertices of lattice tet t (cube t/6, i fastest over (np-1)^3 cubes; local tet t%6).
 ADECO_HD void lattice_tet_vertices(const LatticeView& lat, const PackedCorners& ct, int64_t t, V3<double> v[4]) {
+    if (lat.soup) {
+        const float* p = lat.soup + 12 * t;
+        for (int q = 0; q < 4; ++q) v[q] = mk(double(p[3 * q]), double(p[3 * q + 1]), double(p[3 * q + 2]));
+        return;
+    }
     const int64_t c = t / 6;
     const int lt = int(t - 6 * c);
     const int64_t ncx = lat.np[0] - 1, ncy = lat.np[1] - 1;

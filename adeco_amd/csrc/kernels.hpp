@@ -50,12 +50,14 @@ inline GridGeom make_grid_geom(const float bbox[6], int dims, int scaling = kSca
 // Library-wide cell scaling (adeco_set_cell_scaling); read when a deposit starts.
 int cell_scaling();
 
-// Resident lattice: SoA float vertex coordinates, np[0]*np[1]*np[2] vertices, i fastest.
+// Resident lattice: SoA float vertex coordinates, np[0]*np[1]*np[2] vertices, i fastest.  With
+// `soup` set, tet t is instead the 4 vertices soup[12 t .. 12 t + 11] of a tet soup (AoS xyz).
 struct LatticeView {
     const float* x;
     const float* y;
     const float* z;
     int np[3];
+    const float* soup = nullptr;
 };
 
 // Per-launch work statistics (device counters, filled only by the counting variants).

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/adeco.h"
+#include "batched_impl.hpp"
 #include "deposit_tet.hpp"
 #include "kernels.hpp"
 #include "lattice_impl.hpp"
@@ -46,7 +47,60 @@ __global__ void __launch_bounds__(256) k_minmax(const float* __restrict__ x, con
     if (threadIdx.x < 6) out[6 * blockIdx.x + threadIdx.x] = sm[threadIdx.x][0];
 }
 
+__global__ void __launch_bounds__(256) k_iota_i32(int32_t* out, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = int32_t(i);
+}
+
 }  // namespace
+
+// Batched deposit of a tet soup (already on the device) onto one grid: every tet is one pair of
+// a one-grid candidate space.  Kernel time, partial cells and FP64 recomputes go to g_timings.
+adeco_status deposit_soup_batched(DeviceState& ds, const float* d_tets, int64_t n_tets, const GridGeom& g, double tet_mass,
+                                  int precision, double* d_grid) {
+    g_timings = LastTimings();
+    if (n_tets == 0) return ADECO_OK;
+    if (n_tets >= (int64_t(1) << 31) - 2) return fail(ADECO_EOVERFLOW, "deposit_soup(): too many tets for the batched form");
+    if (g.dims > 1023) return fail(ADECO_EINVAL, "deposit_soup(): the batched form needs dims < 1024");
+    DevBuf ids, desc, work, tiles, ctr;
+    adeco_status st;
+    if ((st = ids.reserve(size_t(n_tets) * 4)) || (st = desc.reserve(512)) ||
+        (st = work.reserve(candidate_space_bytes(n_tets) + 256)) || (st = ctr.reserve(64)))
+        return st;
+    const GridDesc gd = make_grid_desc(g, 0);
+    const int64_t goff[2] = {0, n_tets};
+    GridDesc* d_gd = desc.as<GridDesc>();
+    int64_t* d_goff = reinterpret_cast<int64_t*>(desc.as<char>() + 256);
+    hipLaunchKernelGGL(k_iota_i32, dim3(unsigned((n_tets + 255) / 256)), dim3(256), 0, ds.stream, ids.as<int32_t>(), n_tets);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(d_gd, &gd, sizeof(gd), hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_goff, goff, sizeof(goff), hipMemcpyHostToDevice, ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "soup setup");
+    LatticeView lv{nullptr, nullptr, nullptr, {0, 0, 0}, d_tets};
+    const PackedCorners ct = pack_corners(corner_table(0));
+    CandidateSpace cs;
+    if ((st = build_candidate_space(ds.stream, lv, ct, ids.as<int32_t>(), n_tets, d_goff, 1, d_gd, tet_mass, work.ptr, cs)))
+        return st;
+    if ((st = tiles.reserve(tile_map_bytes(cs.n_cand))) || (st = build_tile_map(ds.stream, cs, tiles.as<int32_t>()))) return st;
+    unsigned* q = ctr.as<unsigned>();
+    unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctr.as<char>() + 16);
+    e = launch_zero_u32(q, 16, ds.stream);
+    const TileSource src{cs.pd, cs.cand_off, cs.tile_first, cs.n_tiles, cs.n_cand, d_gd, d_grid, tet_mass};
+    if (e == hipSuccess) e = hipEventRecord(ds.evk0, ds.stream);
+    if (e == hipSuccess) e = launch_deposit_tiles(ds.stream, precision, src, q, counters, ds.id);
+    if (e == hipSuccess) e = hipEventRecord(ds.evk1, ds.stream);
+    unsigned long long cnt[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt, counters, 16, hipMemcpyDeviceToHost, ds.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);  // the local buffers are freed on return
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ds.evk0, ds.evk1);
+    if (e != hipSuccess) return hip_fail(e, "soup batched deposit");
+    g_timings.main_ms = ms;
+    g_timings.candidates = double(cs.n_cand);
+    g_timings.fallbacks = double(cnt[0]);
+    g_timings.partials = double(cnt[1]);
+    return ADECO_OK;
+}
 
 adeco_status lattice_bbox_device(adeco_lattice_s* L, float out6[6]) {
     DeviceState& ds = *L->ds;
@@ -188,6 +242,8 @@ adeco_status adeco_deposit_soup(int device_id, const float* tets, int64_t n_tets
     g_last_kernel_ms = 0.0;
     if (!tets || !bbox6 || !grid_out || n_tets < 0) return fail(ADECO_EINVAL, "deposit_soup(): invalid argument");
     if (dims < 1 || !valid_bbox(bbox6)) return fail(ADECO_EINVAL, "deposit_soup(): invalid grid");
+    if (mode != ADECO_MODE_PLAIN && mode != ADECO_MODE_BATCHED) return fail(ADECO_EINVAL, "deposit_soup(): unknown mode");
+    if (precision < 0 || precision > 2) return fail(ADECO_EINVAL, "deposit_soup(): unknown precision");
     adeco_status st = check_device(device_id);
     if (st) return st;
     DeviceState& ds = device_state(device_id);
@@ -205,17 +261,19 @@ adeco_status adeco_deposit_soup(int device_id, const float* tets, int64_t n_tets
     if (e == hipSuccess && stats) e = hipMemsetAsync(ds.stats.ptr, 0, kNumStats * sizeof(unsigned long long), ds.stream);
     if (e != hipSuccess) return hip_fail(e, "soup setup");
     if ((st = timed_begin(ds))) return st;
-    e = launch_deposit_soup_plain(ds.pos.as<float>(), n_tets, g, tet_mass, ds.grid.as<double>(),
-                                  stats ? ds.stats.as<unsigned long long>() : nullptr, ds.stream);
-    if (e != hipSuccess) return hip_fail(e, "soup launch");
+    if (mode == ADECO_MODE_BATCHED) {
+        if ((st = deposit_soup_batched(ds, ds.pos.as<float>(), n_tets, g, tet_mass, precision, ds.grid.as<double>()))) return st;
+    } else {
+        e = launch_deposit_soup_plain(ds.pos.as<float>(), n_tets, g, tet_mass, ds.grid.as<double>(),
+                                      stats ? ds.stats.as<unsigned long long>() : nullptr, ds.stream);
+        if (e != hipSuccess) return hip_fail(e, "soup launch");
+    }
     if ((st = timed_end(ds))) return st;
     e = hipMemcpyAsync(grid_out, ds.grid.ptr, ncell * sizeof(double), hipMemcpyDeviceToHost, ds.stream);
     if (e == hipSuccess && stats)
         e = hipMemcpyAsync(stats, ds.stats.ptr, kNumStats * sizeof(unsigned long long), hipMemcpyDeviceToHost, ds.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
     if (e != hipSuccess) return hip_fail(e, "soup download");
-    (void)mode;
-    (void)precision;
     return ADECO_OK;
 }
 

@@ -152,11 +152,6 @@ __global__ void __launch_bounds__(256) k_density_f32(const double* __restrict__ 
         out[i] = float(mass[i] * inv_cell_vol);
 }
 
-__global__ void k_zero_u32(unsigned* p, int n) {
-    const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
 // Request-side grid geometry: the FP64 box of the request with per-axis (or reference) scaling.
 GridGeom roi_geom(const double roi[6], int dims) { return make_grid_geom_d(roi, dims, cell_scaling()); }
 
@@ -464,10 +459,7 @@ void adeco_server_s::run_worker() {
         unsigned long long* counters = reinterpret_cast<unsigned long long*>(d_ctr.as<char>() + 16);
         if (n > 0) {
             e = hipMemcpyAsync(d_entries.ptr, h_entries, size_t(n) * sizeof(int2), hipMemcpyHostToDevice, stream);
-            if (e == hipSuccess) {
-                hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, ctr, 16);
-                e = hipGetLastError();
-            }
+            if (e == hipSuccess) e = launch_zero_u32(ctr, 16, stream);
             if (e == hipSuccess)
                 e = launch_deposit_step(stream, cfg.precision, d_sources.as<TileSource>(), d_entries.as<int2>(), n, ctr,
                                         counters, L->device);

@@ -288,7 +288,10 @@ def main():
         log(f"[bench] octree {tree.num_nodes} nodes / {tree.num_leaves} leaves in {t_oct*1e3:.1f} ms; "
             f"binning {counts.sum()} pairs in {t_bin*1e3:.1f} ms")
 
-        # headline: batched persistent form, FP64
+        # launch shape by the autotuner (untimed), then the headline: batched persistent form, FP64
+        t0 = time.perf_counter()
+        shape = tree.autotune(DIMS, decomp, TET_MASS, A.PREC_FP64, reps=1)
+        log(f"[bench] autotune: {shape} in {time.perf_counter() - t0:.1f}s")
         wall, main_ms, tl = run_steps(tree, args.steps, args.warmup, A.MODE_BATCHED, A.PREC_FP64, decomp)
         step_ms = wall / args.steps * 1e3
         value = lat.num_tets / (wall / args.steps)
@@ -299,6 +302,15 @@ def main():
             f"candidates {tl['candidates']}, partials {tl['partials']}, mass drift {drift:.2e}")
 
         extra = {}
+        # the same deposit captured as a hipGraph and replayed (candidate space built once)
+        with A.DepositGraph(tree, DIMS, decomp, TET_MASS, A.PREC_FP64) as g:
+            g.launch()
+            gms = [g.launch() for _ in range(max(1, args.steps))]
+            gerr = max_rel_err(g.leaf_grids(), grids64, 1e-8 * TET_MASS)
+        extra["graph_replay_ms"] = float(np.mean(gms))
+        extra["graph_replay_tets_per_s"] = lat.num_tets / (float(np.mean(gms)) * 1e-3)
+        extra["graph_vs_eager_max_rel_err"] = gerr
+        log(f"[bench] graph replay: {np.mean(gms):.2f} ms ({extra['graph_replay_tets_per_s']:.3e} tets/s), err {gerr:.1e}")
         if not args.quick:
             wall_p, main_p, _ = run_steps(tree, max(2, args.steps // 2), 1, A.MODE_PLAIN, A.PREC_FP64, decomp)
             plain_tets = lat.num_tets / (wall_p / max(2, args.steps // 2))
@@ -356,7 +368,8 @@ def main():
         "config": {"workload": "adaptive exact deposit, 128^3-particle lattice (12,582,912 Kuhn tets), octree depth<=6, "
                                "split >50000, 16^3-cell leaves, batched persistent FP64 form",
                    "lattice": N, "tets": lat.num_tets, "leaves": int(leaves.size), "leaf_dims": DIMS,
-                   "octree_ms": t_oct * 1e3, "binning_ms": t_bin * 1e3, "mass_drift": drift},
+                   "octree_ms": t_oct * 1e3, "binning_ms": t_bin * 1e3, "mass_drift": drift,
+                   "launch_shape": {"tiles_per_pop": shape["chunk"], "workgroups_per_cu": shape["blocks_per_cu"]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -99,7 +99,9 @@ adeco_status adeco_lattice_deposit_grid(adeco_lattice lat, int decomposition, do
                                         int dims, int mode, int precision, double* grid_out,
                                         uint64_t* stats);
 
-/* Deposits a tet soup (float [n][4][3]) onto one grid (precision study inputs). */
+/* Deposits a tet soup (float [n][4][3]) onto one grid (precision study inputs).  MODE_PLAIN runs
+ * the one-tet-per-thread FP64 kernel; MODE_BATCHED the persistent (tet, cell) kernel in the given
+ * precision, with its kernel time, partial cells and FP64 recomputes in adeco_last_timings. */
 adeco_status adeco_deposit_soup(int device_id, const float* tets, int64_t n_tets, const float* bbox6,
                                 int dims, double tet_mass, int mode, int precision, double* grid_out,
                                 uint64_t* stats);
@@ -159,6 +161,18 @@ adeco_status adeco_get_launch_shape(int* chunk, int* blocks_per_cu);
  * (minimum kernel time of reps runs each), keeps the fastest as the launch shape and returns it. */
 adeco_status adeco_autotune_batched(adeco_octree tree, int dims, int decomposition, double tet_mass, int precision,
                                     int reps, int* best_chunk, int* best_blocks_per_cu, double* best_ms);
+/* hipGraph form of the batched leaf deposit (no reference counterpart: the reference launches per
+ * call, AdDeposit2Grid.cu:229-584).  Create builds the candidate space from the current positions
+ * and leaf binning and captures {grid clear, work-queue reset, persistent deposit launch}; each
+ * launch replays it (synchronous; the graph's time is reported by adeco_last_kernel_ms and
+ * adeco_last_timings).  The graph owns its buffers, including its own leaf grids; destroy it
+ * before the lattice. */
+typedef struct adeco_deposit_graph_s* adeco_deposit_graph;
+adeco_status adeco_deposit_graph_create(adeco_octree tree, int dims, int decomposition, double tet_mass, int precision,
+                                        adeco_deposit_graph* out);
+adeco_status adeco_deposit_graph_launch(adeco_deposit_graph graph);
+adeco_status adeco_deposit_graph_grids(adeco_deposit_graph graph, int64_t first_leaf, int64_t n_leaves, double* out);
+adeco_status adeco_deposit_graph_destroy(adeco_deposit_graph graph);
 /* The calling thread's last leaf deposit: out5 = {total ms, dominant-kernel ms, (tet, cell)
  * candidates, partial cells clipped, FP64 recomputes of the mixed-precision clip}. */
 void adeco_last_timings(double* out5);

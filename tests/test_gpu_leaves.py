@@ -82,3 +82,52 @@ def test_batched_is_repeatable_and_dims_change():
         c = tree.leaf_grids()
     assert O.max_rel_err(c, a, tet_mass=TET_MASS) <= 1e-12
     assert abs(a.sum() - b.sum()) <= 1e-9 * a.sum()
+
+
+def test_graph_replays_match_eager():
+    """The captured graph (grid clear, work-queue reset, persistent launch) replayed several times,
+    with eager deposits of other sizes in between, gives the eager batched result each time."""
+    lat = make_lattice(20, 2.5, seed=12)
+    with A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 1200, 4) as tree:
+        for prec in (A.PREC_FP64, A.PREC_MIXED):
+            tree.deposit(12, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=A.MODE_BATCHED, precision=prec)
+            eager = tree.leaf_grids()
+            with A.DepositGraph(tree, 12, A.DECOMP_KUHN, TET_MASS, prec) as g:
+                for rep in range(3):
+                    print(f"[graph] precision={prec} replay {rep}", flush=True)
+                    g.launch()
+                    got = g.leaf_grids()
+                    assert O.max_rel_err(got, eager, tet_mass=TET_MASS) <= 1e-12
+                    assert A.Octree.last_timings()["partials"] > 0
+                    tree.deposit(8 + 2 * rep, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=A.MODE_BATCHED)
+
+
+def test_autotune_keeps_results():
+    lat = make_lattice(16, 2.0, seed=6)
+    with A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 1500, 3) as tree:
+        tree.deposit(12, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=A.MODE_BATCHED)
+        ref = tree.leaf_grids()
+        saved = A.get_launch_shape()
+        try:
+            pick = tree.autotune(12, A.DECOMP_KUHN, TET_MASS, A.PREC_FP64, reps=1)
+            assert pick["chunk"] in (1, 2, 4, 8, 16, 32) and pick["blocks_per_cu"] in (1, 2, 3, 4)
+            assert A.get_launch_shape() == (pick["chunk"], pick["blocks_per_cu"])
+            tree.deposit(12, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=A.MODE_BATCHED)
+            assert O.max_rel_err(tree.leaf_grids(), ref, tet_mass=TET_MASS) <= 1e-12
+        finally:
+            A.set_launch_shape(*saved)
+
+
+def test_c1_single_grid_matches_oracle():
+    """The CPU-runnable configuration: 32^3 lattice at rms 1 spacing, 6 Kuhn tets per cube, tet
+    mass 1/6, one 64^3 grid over the root (an octree that does not refine)."""
+    lat = make_lattice(32, 1.0, seed=1)
+    with O.cell_scaling(O.EXACT), A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 10 ** 9, 6) as tree:
+        assert tree.num_leaves == 1
+        want = O.deposit_lattice(lat, tree.root_box_f64(), 64, decomp=A.DECOMP_KUHN, tet_mass=TET_MASS, precision=3,
+                                 nthreads=8)
+        for mode in (A.MODE_PLAIN, A.MODE_BATCHED):
+            tree.deposit(64, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=mode)
+            got = tree.leaf_grids()[0]
+            assert O.max_rel_err(got, want, tet_mass=TET_MASS) <= REL
+            assert abs(got.sum() - 32 ** 3) <= 1e-9 * 32 ** 3
