@@ -100,18 +100,46 @@ def _apply_infall(px, py, pz, box, count, rng):
 
     Centres are uniform in the box, R ~ U[2, 8] Mpc/h and A ~ U[0.8, 1.6] (A > 1 inverts the
     tets near the centre).  Distances use the minimum image so periodic images move together.
+    Each centre only touches the vertices within R of it, so the update runs over a window of
+    lattice indices that provably holds them (unperturbed position within R + the largest current
+    displacement, with periodic wrap); every other vertex has w = 0.
     """
     centres = rng.uniform(0.0, box, size=(count, 3))
     radii = rng.uniform(2.0, 8.0, size=count)
     amps = rng.uniform(0.8, 1.6, size=count)
     pos = [px.copy(), py.copy(), pz.copy()]
+    nv = px.shape[0]  # arrays are [k, j, i] with nv = n + 1 vertices per axis
+    n = nv - 1
+    delta = box / n
+    q = np.arange(nv) * delta
+    # largest |position - unperturbed position| over all vertices and axes; only windowed vertices
+    # move, so it is updated from them
+    smax = max(float(np.max(np.abs(pos[0] - q[None, None, :]))), float(np.max(np.abs(pos[1] - q[None, :, None]))),
+               float(np.max(np.abs(pos[2] - q[:, None, None]))))
     for c, r, a in zip(centres, radii, amps):
-        d = [pos[i] - c[i] for i in range(3)]
+        reach = r + smax + delta
+        win = []
+        for a_ in range(3):
+            lo = int(np.floor((c[a_] - reach) / delta)) - 1
+            hi = int(np.ceil((c[a_] + reach) / delta)) + 1
+            if hi - lo + 1 >= n:
+                idx = np.arange(nv)
+            else:
+                idx = np.unique(np.arange(lo, hi + 1) % n)
+                if 0 in idx:
+                    idx = np.append(idx, n)  # vertex n is vertex 0 one box over
+            win.append(idx)
+        sel = np.ix_(win[2], win[1], win[0])
+        sub = [pos[i][sel] for i in range(3)]
+        d = [sub[i] - c[i] for i in range(3)]
         d = [di - box * np.round(di / box) for di in d]
         rr = np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2) / r
         w = np.where(rr < 1.0, (1.0 - rr * rr) ** 2, 0.0)
+        qs = (q[win[0]][None, None, :], q[win[1]][None, :, None], q[win[2]][:, None, None])
         for i in range(3):
-            pos[i] = pos[i] - a * d[i] * w
+            moved = sub[i] - a * d[i] * w
+            pos[i][sel] = moved
+            smax = max(smax, float(np.max(np.abs(moved - qs[i]))))
     return pos[0], pos[1], pos[2]
 
 

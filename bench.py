@@ -41,6 +41,9 @@ DIMS = 16
 TET_MASS = 1.0 / 6.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (the roof of an FP64 VALU-bound kernel)
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of k_deposit_waves<FP64> on this workload: rocprofv3 FETCH_SIZE x 2 +
+# WRITE_SIZE x 1 (each scaled by what it reads for a 1 GiB copy), profiles/deposit_waves_c2_pmc.md
+TRAFFIC_BYTES = None
 
 
 def log(*a):
@@ -139,22 +142,16 @@ def cpu_baseline(lat, tree_leaves_order, budget_s=20.0):
                       f"{threads} threads, {wall:.2f} s"}
 
 
-def algorithmic_flops(lat, tree, leaves, decomp, n_sample=24):
-    """Reference-algorithm FP64 operation count per (leaf, tet) pair, from the oracle's counting
-    mode on a sample of leaves; returns (flops per pair, pairs per step)."""
-    from tests import _oracle as O  # checker side only
-    counts = tree.bin(decomp)
-    boxes = tree.leaf_boxes()
-    pick = np.linspace(0, leaves.size - 1, n_sample).astype(int)
-    flops = 0
-    pairs = 0
-    for l in pick:
-        ids = tree.leaf_tets(int(l))
-        _, st = O.deposit_lattice(lat, boxes[l], DIMS, decomp=decomp, tet_mass=TET_MASS, tet_ids=ids, precision=2,
-                                  stats=True)
-        flops += int(st[12])
-        pairs += ids.size
-    return flops / max(pairs, 1), int(counts.sum())
+def algorithmic_flops():
+    """Reference-algorithm FP64 operations of one step of this workload, counted once by the
+    oracle's counting instantiation and committed as tests/golden/c2_flops.json (written by
+    tests/golden/make_flops.py); returns (flops, pairs) or None when the fixture is absent."""
+    path = os.path.join(ROOT, "tests", "golden", "c2_flops.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return float(d["flops"]), int(d["pairs"])
 
 
 SERVE_N = 256
@@ -250,6 +247,104 @@ def serve_study(rates, count, seed=5, max_inflight=16):
     return out
 
 
+def grid_errors(got, ref, floor):
+    """Per-cell max relative error (floor for near-empty cells), L1 error relative to the total, and
+    the total-mass drift of `got` against `ref`."""
+    got = np.asarray(got, np.float64).ravel()
+    ref = np.asarray(ref, np.float64).ravel()
+    tot = float(np.sum(np.abs(ref)))
+    return {"max_rel": float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), floor))),
+            "l1_rel": float(np.sum(np.abs(got - ref)) / tot), "mass_drift": float(abs(got.sum() - ref.sum()) / tot)}
+
+
+def precision_study(n_degenerate=1_000_000, seed=4):
+    """Precision modes on the 128^3 workload's leaves and on near-degenerate tets (edge ratios
+    1e3..1e6, volumes down to ~1e-12 cell^3) deposited as a soup on a 128^3 grid over the box:
+    FP64; FP32 clip with FP64 orientation/volume and FP64 recompute where the FP32 error bound
+    fails (MIXED); pure FP32 clip.  Errors are against the FP64 form, itself held to the
+    extended-precision oracle by the GPU parity tests."""
+    from adeco_amd.lattice import make_degenerate_tets
+    out = {"leaves": {}, "degenerate": {}}
+    floor = 1e-8 * TET_MASS
+    lat = make_lattice(N, RMS, seed=SEED)
+    with A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, MAX_TETS, MAX_LEVEL) as tree:
+        tree.bin(A.DECOMP_KUHN)
+        ref = None
+        for name, prec in (("fp64", A.PREC_FP64), ("mixed", A.PREC_MIXED), ("fp32", A.PREC_FP32)):
+            tree.deposit(DIMS, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=A.MODE_BATCHED, precision=prec)
+            tl = A.Octree.last_timings()
+            g = tree.leaf_grids()
+            rec = {"kernel_ms": tl["main_ms"], "partial_cells": tl["partials"],
+                   "fp64_recompute_fraction": tl["fallbacks"] / max(tl["partials"], 1)}
+            if ref is None:
+                ref = g
+                rec["mass_drift_vs_exact"] = abs(g.sum() - N ** 3) / N ** 3
+            else:
+                rec.update(grid_errors(g, ref, floor))
+            out["leaves"][name] = rec
+            log(f"[precision] leaves {name}: {rec}")
+    tets = make_degenerate_tets(n_degenerate, seed=seed)
+    box = np.array([0, 100, 0, 100, 0, 100], np.float32)
+    ref = A.deposit_soup(tets, box, 128, TET_MASS, mode=A.MODE_BATCHED, precision=A.PREC_FP64)
+    tl = A.Octree.last_timings()
+    out["degenerate"]["fp64"] = {"kernel_ms": tl["main_ms"], "partial_cells": tl["partials"]}
+    plain = A.deposit_soup(tets, box, 128, TET_MASS, mode=A.MODE_PLAIN)
+    out["degenerate"]["fp64_plain_kernel"] = grid_errors(plain, ref, floor)
+    for name, prec in (("mixed", A.PREC_MIXED), ("fp32", A.PREC_FP32)):
+        g = A.deposit_soup(tets, box, 128, TET_MASS, mode=A.MODE_BATCHED, precision=prec)
+        tl = A.Octree.last_timings()
+        rec = {"kernel_ms": tl["main_ms"], "partial_cells": tl["partials"],
+               "fp64_recompute_fraction": tl["fallbacks"] / max(tl["partials"], 1)}
+        rec.update(grid_errors(g, ref, floor))
+        out["degenerate"][name] = rec
+        log(f"[precision] degenerate {name}: {rec}")
+    out["degenerate"]["tets"] = int(n_degenerate)
+    return out
+
+
+def heavy_tail_study(dims_list=(16, 32), n=256, steps=3):
+    """The heavy-tailed batched workload: a 256^3 lattice (100,663,296 Kuhn tets) at rms 4 spacings
+    with 64 seeded spherical infall centres (radius 2-8 Mpc/h), the reference octree (depth <= 6,
+    split above 50,000 estimated tets) and 16^3 or 32^3-cell leaves.  The batched form (autotuned
+    launch shape; eager and hipGraph replay) against the plain per-leaf launches."""
+    t0 = time.perf_counter()
+    log(f"[c3] generating the {n}^3 lattice with 64 infall centres")
+    lat = make_lattice(n, 4.0, seed=3, infall_centres=64)
+    log(f"[c3] lattice {n}^3 with infall generated in {time.perf_counter() - t0:.1f}s")
+    out = {"lattice": n, "tets": lat.num_tets, "runs": []}
+    with A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, MAX_TETS, MAX_LEVEL) as tree:
+        counts = tree.bin(A.DECOMP_KUHN)
+        nz = counts[counts > 0]
+        out["leaves"] = int(tree.num_leaves)
+        out["binned_pairs"] = int(counts.sum())
+        out["leaf_tets_min_median_max"] = [int(nz.min()), int(np.median(nz)), int(nz.max())]
+        out["leaf_tets_p99_over_median"] = float(np.percentile(nz, 99) / np.median(nz))
+        log(f"[c3] {tree.num_leaves} leaves, pairs {counts.sum()}, per-leaf tets min/median/max "
+            f"{out['leaf_tets_min_median_max']}")
+        inside = None
+        for dims in dims_list:
+            shape = tree.autotune(dims, A.DECOMP_KUHN, TET_MASS, A.PREC_FP64, reps=1)
+            wall, main_ms, tl = run_steps(tree, steps, 1, A.MODE_BATCHED, A.PREC_FP64, A.DECOMP_KUHN)
+            g = tree.leaf_grids()
+            total = float(g.sum())
+            del g
+            with A.DepositGraph(tree, dims, A.DECOMP_KUHN, TET_MASS, A.PREC_FP64) as gr:
+                gr.launch()
+                gms = float(np.mean([gr.launch() for _ in range(steps)]))
+            wall_p, main_p, _ = run_steps(tree, 1, 0, A.MODE_PLAIN, A.PREC_FP64, A.DECOMP_KUHN)
+            rec = {"leaf_dims": dims, "launch_shape": shape, "batched_ms_per_step": wall / steps * 1e3,
+                   "batched_kernel_ms": main_ms, "batched_tets_per_s": lat.num_tets / (wall / steps),
+                   "graph_replay_ms": gms, "graph_tets_per_s": lat.num_tets / (gms * 1e-3),
+                   "plain_ms_per_step": wall_p * 1e3, "plain_tets_per_s": lat.num_tets / wall_p,
+                   "batched_over_plain": wall_p / (wall / steps), "candidates": tl["candidates"],
+                   "partials": tl["partials"], "deposited_mass": total}
+            out["runs"].append(rec)
+            log(f"[c3] {dims}^3 leaves: batched {rec['batched_ms_per_step']:.1f} ms/step "
+                f"({rec['batched_tets_per_s']:.3e} tets/s), graph {gms:.1f} ms, plain {wall_p * 1e3:.0f} ms "
+                f"-> {rec['batched_over_plain']:.1f}x; mass {total:.6f}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,12 +353,26 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the plain and mixed comparison runs")
     ap.add_argument("--serve", action="store_true", help="run only the ROI serving study (256^3 lattice)")
+    ap.add_argument("--precision", action="store_true", help="run only the precision-mode study")
+    ap.add_argument("--heavy-tail", action="store_true", help="run only the 256^3 heavy-tailed batched study")
     ap.add_argument("--serve-rates", default="20,50,100,200")
     ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: every box has one MI355X; run with --gpus 1"}))
         sys.exit(2)
+    if args.heavy_tail:
+        res = heavy_tail_study()
+        print(json.dumps({"metric": "tets/s exact deposit, batched vs plain per-leaf launches", "unit": "tets/s",
+                          "higher_is_better": True, "n_gpus": 1, "dtype": "f64",
+                          "data": "synthetic (256^3 lattice, rms 4 spacings + 64 infall centres)", **res}))
+        return
+    if args.precision:
+        res = precision_study()
+        print(json.dumps({"metric": "per-cell max rel error, L1 error and mass drift of the precision modes vs FP64",
+                          "unit": "relative", "higher_is_better": False, "n_gpus": 1, "dtype": "f64/f32",
+                          "data": "synthetic (128^3 lattice leaves; 1M near-degenerate tets)", **res}))
+        return
     if args.serve:
         res = serve_study([float(r) for r in args.serve_rates.split(",")], args.serve_requests)
         print(json.dumps({"metric": "p50/p99 ROI latency (s) and tets/s, continuous batching vs serial plain GPU",
@@ -332,16 +441,16 @@ def main():
             tree.bin(decomp)
 
         roofline = None
-        try:
-            flops_per_pair, n_pairs = algorithmic_flops(lat, tree, leaves, decomp)
-            achieved = flops_per_pair * n_pairs / (main_ms * 1e-3) / 1e12
+        fl = algorithmic_flops()
+        if fl is not None:
+            achieved = fl[0] / (main_ms * 1e-3) / 1e12
             roofline = {"bound": "mfma", "pipe": "VALU FP64 (clipping geometry; no matrix work)",
                         "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                        "flops_per_pair": flops_per_pair, "pairs_per_launch": n_pairs,
-                        "kernel_ms": main_ms, "kernel": "k_deposit_waves<FP64>"}
-        except Exception as exc:  # the oracle is absent: report the roofline as unmeasured
-            log(f"[bench] roofline skipped: {exc}")
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": TRAFFIC_BYTES,
+                        "flops_per_launch": fl[0], "pairs_counted": fl[1], "kernel_ms": main_ms,
+                        "kernel": "k_deposit_waves<FP64>"}
+        else:
+            log("[bench] roofline skipped: tests/golden/c2_flops.json is missing")
 
         cpu = None
         if not args.no_cpu_baseline:
