@@ -43,20 +43,20 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (the roof of an FP64 VALU-bou
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of k_deposit_waves<FP64> on this workload: rocprofv3 FETCH_SIZE x 2 +
 # WRITE_SIZE x 1 (each scaled by what it reads for a 1 GiB copy), profiles/deposit_waves_c2_pmc.md
-TRAFFIC_BYTES = None
+TRAFFIC_BYTES = 4.809e9
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_steps(tree, steps, warmup, mode, precision, decomp):
+def run_steps(tree, steps, warmup, mode, precision, decomp, dims=DIMS):
     for _ in range(warmup):
-        tree.deposit(DIMS, decomposition=decomp, tet_mass=TET_MASS, mode=mode, precision=precision)
+        tree.deposit(dims, decomposition=decomp, tet_mass=TET_MASS, mode=mode, precision=precision)
     wall, main_ms, tl = [], [], None
     for _ in range(steps):
         t0 = time.perf_counter()
-        tree.deposit(DIMS, decomposition=decomp, tet_mass=TET_MASS, mode=mode, precision=precision)
+        tree.deposit(dims, decomposition=decomp, tet_mass=TET_MASS, mode=mode, precision=precision)
         wall.append(time.perf_counter() - t0)
         tl = A.Octree.last_timings()
         main_ms.append(tl["main_ms"])
@@ -324,14 +324,14 @@ def heavy_tail_study(dims_list=(16, 32), n=256, steps=3):
         inside = None
         for dims in dims_list:
             shape = tree.autotune(dims, A.DECOMP_KUHN, TET_MASS, A.PREC_FP64, reps=1)
-            wall, main_ms, tl = run_steps(tree, steps, 1, A.MODE_BATCHED, A.PREC_FP64, A.DECOMP_KUHN)
+            wall, main_ms, tl = run_steps(tree, steps, 1, A.MODE_BATCHED, A.PREC_FP64, A.DECOMP_KUHN, dims)
             g = tree.leaf_grids()
             total = float(g.sum())
             del g
             with A.DepositGraph(tree, dims, A.DECOMP_KUHN, TET_MASS, A.PREC_FP64) as gr:
                 gr.launch()
                 gms = float(np.mean([gr.launch() for _ in range(steps)]))
-            wall_p, main_p, _ = run_steps(tree, 1, 0, A.MODE_PLAIN, A.PREC_FP64, A.DECOMP_KUHN)
+            wall_p, main_p, _ = run_steps(tree, 1, 0, A.MODE_PLAIN, A.PREC_FP64, A.DECOMP_KUHN, dims)
             rec = {"leaf_dims": dims, "launch_shape": shape, "batched_ms_per_step": wall / steps * 1e3,
                    "batched_kernel_ms": main_ms, "batched_tets_per_s": lat.num_tets / (wall / steps),
                    "graph_replay_ms": gms, "graph_tets_per_s": lat.num_tets / (gms * 1e-3),
