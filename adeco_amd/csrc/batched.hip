@@ -114,10 +114,13 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
     const int dm1 = g.dims - 1;
     unsigned n = 1;
     for (int a = 0; a < 3; ++a) {
-        // covered range of the tet AABB (clamping the overlap to the grid first changes nothing
-        // after the clamp of the indices)
-        int l = int(floor((mn[a] - g.gmin[a]) / g.delta[a]));
-        int h = int(ceil((mx[a] - g.gmin[a]) / g.delta[a]));
+        // cells whose box can strictly overlap the tet AABB.  The reference covers [floor, ceil]
+        // (AdDepositTets2Grid.h:530-539), one cell more per axis than can pass its strict test
+        // (AdDepositTets2Grid.h:598); the range here stops at floor, widened by 1e-9 of a cell each
+        // way so that rounding in the division cannot drop a touching cell.  Clamping the overlap to
+        // the grid first changes nothing after the clamp of the indices.
+        int l = int(floor((mn[a] - g.gmin[a]) / g.delta[a] - 1e-9));
+        int h = int(floor((mx[a] - g.gmin[a]) / g.delta[a] + 1e-9));
         l = l <= 0 ? 0 : (l >= dm1 ? dm1 : l);
         h = h <= 0 ? 0 : (h >= dm1 ? dm1 : h);
         const unsigned e = unsigned(h - l + 1);

@@ -1,7 +1,7 @@
 // Per-tet exact deposit onto one uniform grid, FP64 throughout (host + device).
 //
 // Follows deposit_tet_2_grid (AdDepositTets2Grid.h:509-665): tet AABB and its strict overlap with
-// the grid, the covered cell range [floor, ceil] clamped to the grid, the signed volume (zero ->
+// the grid, the covered cell range clamped to the grid (the cells its strict test can pass), the signed volume (zero ->
 // skip; negative -> vertex order (v3, v1, v2, v0)), then per covered cell the strict AABB cull,
 // the scaled tet s = scale_a * (src - cell_min) per axis, the classification and the mass
 //   tet in cell  -> m
@@ -38,8 +38,10 @@ ADECO_HD void deposit_tet_f64(const V3<double> v[4], const GridGeom& g, double t
     for (int a = 0; a < 3; ++a) {
         const double omn = tmn[a] < g.gmin[a] ? g.gmin[a] : tmn[a];
         const double omx = tmx[a] < g.gmax[a] ? tmx[a] : g.gmax[a];
-        int l = int(floor((omn - g.gmin[a]) / g.delta[a]));
-        int h = int(ceil((omx - g.gmin[a]) / g.delta[a]));
+        // [floor, floor] widened by 1e-9 of a cell: the cells the strict overlap test can pass
+        // (the reference's ceil adds one that never does)
+        int l = int(floor((omn - g.gmin[a]) / g.delta[a] - 1e-9));
+        int h = int(floor((omx - g.gmin[a]) / g.delta[a] + 1e-9));
         lo[a] = l <= 0 ? 0 : (l >= dm1 ? dm1 : l);
         hi[a] = h <= 0 ? 0 : (h >= dm1 ? dm1 : h);
     }
