@@ -9,9 +9,10 @@
 // 16-case lookup table through six faces with 10-point scratch arrays per level, which on a GPU
 // means dynamically indexed private memory):
 //   * the x- and y-faces are applied only when they actually cut the tet; each cut turns a tet
-//     into one tet (1 vertex inside), or a triangular prism (2 or 3 inside) written as a 6-point
-//     sequence whose 3 sub-tets are the 4 consecutive points starting at 0, 1 and 2, so every
-//     sub-tet is addressed with selects, never a table lookup;
+//     into one tet (1 vertex inside), into the tet minus the corner tet beyond the plane (3
+//     inside: two signed pieces), or into a wedge of 3 tets (2 inside), written as a point
+//     sequence whose pieces are windows of 4 consecutive points, so every piece is addressed with
+//     selects, never a table lookup;
 //   * the z-slab of each remaining sub-tet is integrated in closed form: the fraction of a tet
 //     beyond a plane, from the signed distances d_i of its vertices, is
 //       1 vertex beyond (a = d > 0, others b_j >= 0 on the near side):  a^3 / prod_j (a + b_j)
@@ -194,10 +195,11 @@ ADECO_HD R zslab_vol6(const V3<R>& a, const V3<R>& b, const V3<R>& c, const V3<R
 
 // One clip stage: the part of tet t on the inside of plane (axis, value c, keep >= c when ge)
 // written as a point sequence seq[0..5] with `count` sub-tets seq[k..k+3], k < count.
-template <class R>
+template <class R, int NS = 7>
 struct Stage {
-    V3<R> s[6];
-    int count;
+    V3<R> s[NS];
+    int count;     // pieces: windows 0..count-1, except that piece 1 is window 3 when corner is set
+    bool corner;   // 3 vertices inside: piece 0 = the input tet (+), piece 1 = the corner beyond (-)
 };
 
 template <class R>
@@ -208,8 +210,8 @@ ADECO_HD V3<R> cut_point(const V3<R>& in, R din, const V3<R>& out, R dout, int a
     return p;
 }
 
-template <class R>
-ADECO_HD void clip_stage(const V3<R> t[4], int axis, R c, bool ge, Stage<R>& st) {
+template <class R, int NS = 7>
+ADECO_HD void clip_stage(const V3<R> t[4], int axis, R c, bool ge, Stage<R, NS>& st, bool use_signed = true) {
     R d[4];
     for (int i = 0; i < 4; ++i) {
         const R x = comp(t[i], axis);
@@ -218,8 +220,10 @@ ADECO_HD void clip_stage(const V3<R> t[4], int axis, R c, bool ge, Stage<R>& st)
     const int m = int(d[0] > R(0)) | (int(d[1] > R(0)) << 1) | (int(d[2] > R(0)) << 2) | (int(d[3] > R(0)) << 3);
     const int n = (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1) + ((m >> 3) & 1);
     // every path writes all six slots, so the sequence stays in registers
+    st.corner = false;
     if (n == 0 || n == 4) {
         st.s[0] = t[0]; st.s[1] = t[1]; st.s[2] = t[2]; st.s[3] = t[3]; st.s[4] = t[3]; st.s[5] = t[3];
+        if (NS == 7) st.s[NS - 1] = t[3];
         st.count = n == 4 ? 1 : 0;
         return;
     }
@@ -244,12 +248,21 @@ ADECO_HD void clip_stage(const V3<R> t[4], int axis, R c, bool ge, Stage<R>& st)
         st.s[3] = cut_point(p[0], dd[0], p[3], dd[3], axis, c);
         st.s[4] = st.s[3];
         st.s[5] = st.s[3];
+        if (NS == 7) st.s[NS - 1] = st.s[3];
         st.count = 1;
-    } else if (n == 3) {  // prism (a,b,c) - (p_ao, p_bo, p_co)
+    } else if (n == 3 && NS == 7 && use_signed) {  // the tet (a,b,c,d) minus the corner (d, p_ad, p_bd, p_cd)
+        st.s[0] = p[0]; st.s[1] = p[1]; st.s[2] = p[2]; st.s[3] = p[3];
+        st.s[4] = cut_point(p[0], dd[0], p[3], dd[3], axis, c);
+        st.s[5] = cut_point(p[1], dd[1], p[3], dd[3], axis, c);
+        st.s[NS - 1] = cut_point(p[2], dd[2], p[3], dd[3], axis, c);
+        st.count = 2;
+        st.corner = true;
+    } else if (n == 3) {  // prism (a,b,c) - (p_ad, p_bd, p_cd): 3 tets
         st.s[0] = p[0]; st.s[1] = p[1]; st.s[2] = p[2];
         st.s[3] = cut_point(p[0], dd[0], p[3], dd[3], axis, c);
         st.s[4] = cut_point(p[1], dd[1], p[3], dd[3], axis, c);
         st.s[5] = cut_point(p[2], dd[2], p[3], dd[3], axis, c);
+        if (NS == 7) st.s[NS - 1] = st.s[5];
         st.count = 3;
     } else {  // wedge (a, p_ac, p_ad) - (b, p_bc, p_bd)
         st.s[0] = p[0];
@@ -258,6 +271,7 @@ ADECO_HD void clip_stage(const V3<R> t[4], int axis, R c, bool ge, Stage<R>& st)
         st.s[3] = p[1];
         st.s[4] = cut_point(p[1], dd[1], p[2], dd[2], axis, c);
         st.s[5] = cut_point(p[1], dd[1], p[3], dd[3], axis, c);
+        if (NS == 7) st.s[NS - 1] = st.s[5];
         st.count = 3;
     }
 }
@@ -265,12 +279,23 @@ ADECO_HD void clip_stage(const V3<R> t[4], int axis, R c, bool ge, Stage<R>& st)
 // sub-tet k (0..2) = seq[k..k+3]; written out slot by slot so every index is a constant and the
 // sequence never leaves registers
 template <class R>
-ADECO_HD void take(const Stage<R>& st, int k, V3<R> out[4]) {
+ADECO_HD R take(const Stage<R, 7>& st, int k, V3<R> out[4]) {
+    const bool k0 = k == 0, k1 = k == 1 && !st.corner, k3 = k == 1 && st.corner;
+    out[0] = sel(k0, st.s[0], sel(k1, st.s[1], sel(k3, st.s[3], st.s[2])));
+    out[1] = sel(k0, st.s[1], sel(k1, st.s[2], sel(k3, st.s[4], st.s[3])));
+    out[2] = sel(k0, st.s[2], sel(k1, st.s[3], sel(k3, st.s[5], st.s[4])));
+    out[3] = sel(k0, st.s[3], sel(k1, st.s[4], sel(k3, st.s[6], st.s[5])));
+    return k3 ? R(-1) : R(1);
+}
+
+template <class R>
+ADECO_HD R take(const Stage<R, 6>& st, int k, V3<R> out[4]) {
     const bool k0 = k == 0, k1 = k == 1;
     out[0] = sel(k0, st.s[0], sel(k1, st.s[1], st.s[2]));
     out[1] = sel(k0, st.s[1], sel(k1, st.s[2], st.s[3]));
     out[2] = sel(k0, st.s[2], sel(k1, st.s[3], st.s[4]));
     out[3] = sel(k0, st.s[3], sel(k1, st.s[4], st.s[5]));
+    return R(1);
 }
 
 // Which of the faces x>=0, x<=1, y>=0, y<=1 cut the tet (bit f set), or -1 if the tet lies
@@ -311,22 +336,73 @@ struct ClipCounters {
     uint32_t stages, subtets;
 };
 
-// z-slab volumes of the sub-tets of one stage
-template <class R>
-ADECO_HD R stage_zslab(const Stage<R>& st, ClipCounters* cnt) {
+// signed z-slab volumes of the pieces of one stage; *sabs accumulates their magnitudes
+template <class R, int NS>
+ADECO_HD R stage_zslab(const Stage<R, NS>& st, R g, R* sabs, ClipCounters* cnt) {
     if (cnt) cnt->subtets += uint32_t(st.count);
     R v = R(0);
     for (int k = 0; k < st.count; ++k) {
         V3<R> q[4];
-        take(st, k, q);
-        v += zslab_vol6(q[0], q[1], q[2], q[3]);
+        const R gk = g * take(st, k, q);
+        const R w = zslab_vol6(q[0], q[1], q[2], q[3]);
+        v += gk * w;
+        *sabs += w;
     }
     return v;
 }
 
-// 6 x |t ∩ [0,1]^3| (t in cell units, any orientation).
-template <class R>
-ADECO_HD R clip_unit_cube_vol6(const V3<R> t[4], ClipCounters* cnt = nullptr) {
+// The clip tree of t over its cutting x/y faces with the z slab in closed form at the leaves.
+// use_signed: a cut that leaves 3 vertices inside gives the whole tet (+) and the corner beyond the
+// plane (-) instead of 3 prism tets; *sabs returns the sum of the magnitudes of all leaf terms.
+// One instance serves both trees, so a kernel holds the clip code once.
+template <class R, int NS = 7>
+ADECO_HD R clip_tree_vol6(const V3<R> t[4], int act, int nact, bool use_signed, R* sabs, ClipCounters* cnt) {
+    int ax;
+    R c;
+    bool ge;
+    Stage<R, NS> s1, s2, s3, s4;
+    V3<R> q1[4], q2[4], q3[4];
+    *sabs = R(0);
+    face_of(act, 0, ax, c, ge);
+    clip_stage(t, ax, c, ge, s1, use_signed);
+    if (cnt) cnt->stages += 1;
+    if (nact == 1) return stage_zslab(s1, R(1), sabs, cnt);
+    R vol = R(0);
+    for (int k1 = 0; k1 < s1.count; ++k1) {
+        const R g1 = take(s1, k1, q1);
+        face_of(act, 1, ax, c, ge);
+        clip_stage(q1, ax, c, ge, s2, use_signed);
+        if (cnt) cnt->stages += 1;
+        if (nact == 2) {
+            vol += stage_zslab(s2, g1, sabs, cnt);
+            continue;
+        }
+        for (int k2 = 0; k2 < s2.count; ++k2) {
+            const R g2 = g1 * take(s2, k2, q2);
+            face_of(act, 2, ax, c, ge);
+            clip_stage(q2, ax, c, ge, s3, use_signed);
+            if (cnt) cnt->stages += 1;
+            if (nact == 3) {
+                vol += stage_zslab(s3, g2, sabs, cnt);
+                continue;
+            }
+            for (int k3 = 0; k3 < s3.count; ++k3) {
+                const R g3 = g2 * take(s3, k3, q3);
+                face_of(act, 3, ax, c, ge);
+                clip_stage(q3, ax, c, ge, s4, use_signed);
+                if (cnt) cnt->stages += 1;
+                vol += stage_zslab(s4, g3, sabs, cnt);
+            }
+        }
+    }
+    return vol;
+}
+
+// 6 x |t ∩ [0,1]^3| by the signed (use_signed) or the exact tree; *sabs as above (0 when t needs
+// no clip).  NS = 6 compiles the exact tree alone (use_signed is then ignored).
+template <class R, int NS = 7>
+ADECO_HD R clip_any_vol6(const V3<R> t[4], bool use_signed, R* sabs, ClipCounters* cnt = nullptr) {
+    *sabs = R(0);
     const int act = active_xy_faces(t);
     if (act < 0) return R(0);
     const int nact = (act & 1) + ((act >> 1) & 1) + ((act >> 2) & 1) + ((act >> 3) & 1);
@@ -334,44 +410,40 @@ ADECO_HD R clip_unit_cube_vol6(const V3<R> t[4], ClipCounters* cnt = nullptr) {
         if (cnt) cnt->subtets += 1;
         return zslab_vol6(t[0], t[1], t[2], t[3]);
     }
-    int ax;
-    R c;
-    bool ge;
-    Stage<R> s1, s2, s3, s4;
-    V3<R> q1[4], q2[4], q3[4];
-    face_of(act, 0, ax, c, ge);
-    clip_stage(t, ax, c, ge, s1);
-    if (cnt) cnt->stages += 1;
-    if (nact == 1) return stage_zslab(s1, cnt);
-    R vol = R(0);
-    for (int k1 = 0; k1 < s1.count; ++k1) {
-        take(s1, k1, q1);
-        face_of(act, 1, ax, c, ge);
-        clip_stage(q1, ax, c, ge, s2);
-        if (cnt) cnt->stages += 1;
-        if (nact == 2) {
-            vol += stage_zslab(s2, cnt);
-            continue;
-        }
-        for (int k2 = 0; k2 < s2.count; ++k2) {
-            take(s2, k2, q2);
-            face_of(act, 2, ax, c, ge);
-            clip_stage(q2, ax, c, ge, s3);
-            if (cnt) cnt->stages += 1;
-            if (nact == 3) {
-                vol += stage_zslab(s3, cnt);
-                continue;
-            }
-            for (int k3 = 0; k3 < s3.count; ++k3) {
-                take(s3, k3, q3);
-                face_of(act, 3, ax, c, ge);
-                clip_stage(q3, ax, c, ge, s4);
-                if (cnt) cnt->stages += 1;
-                vol += stage_zslab(s4, cnt);
-            }
-        }
+    return clip_tree_vol6<R, NS>(t, act, nact, use_signed, sabs, cnt);
+}
+
+// Signed tree of a cell-local tet with its noise measure: returns 6 x |t ∩ [0,1]^3| and sets *sabs
+// to the sum of the magnitudes of the terms it was summed from.
+template <class R>
+ADECO_HD R clip_signed_vol6(const V3<R> t[4], R* sabs, ClipCounters* cnt = nullptr) {
+    return clip_any_vol6(t, true, sabs, cnt);
+}
+
+// Tree of non-overlapping pieces: every term is positive, no cancellation.
+template <class R>
+ADECO_HD R clip_exact_vol6(const V3<R> t[4], ClipCounters* cnt = nullptr) {
+    R sabs;
+    return clip_any_vol6<R, 6>(t, false, &sabs, cnt);
+}
+
+// A signed result below 1e-6 of the magnitudes it was summed from may be cancellation noise (a few
+// eps of those magnitudes); such a cell is recomputed with the exact tree.
+template <class R>
+ADECO_HD bool signed_is_noisy(R v, R sabs) { return !(v >= R(1e-6) * sabs); }
+
+// 6 x |t ∩ [0,1]^3| (t in cell units, any orientation): the signed tree (fewer pieces), recomputed
+// with the exact tree where it may be noise.
+// The loop keeps one instance of the tree in the caller.
+template <class R>
+ADECO_HD R clip_unit_cube_vol6(const V3<R> t[4], ClipCounters* cnt = nullptr) {
+    bool use_signed = true;
+    for (;;) {
+        R sabs;
+        const R v = clip_any_vol6(t, use_signed, &sabs, cnt);
+        if (!use_signed || !signed_is_noisy(v, sabs)) return v;
+        use_signed = false;
     }
-    return vol;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -22,13 +22,18 @@
 //                       closed-form z axis, and whether the first cut leaves 1 or 3 sub-tets).  A
 //                       queue that reaches 64 entries is clipped by the whole wave, so the lanes of
 //                       one clip follow nearly the same path through the nested clip.
-// Masses go to the grids with FP64 global atomics.  Precision modes for the clip:
+// The clip uses the signed tree (a cut leaving 3 vertices inside gives the tet minus the corner
+// beyond the plane: fewer pieces); a result below 1e-6 of the magnitudes it was summed from may be
+// cancellation noise, so that cell goes to a tenth per-wave queue whose waves clip it with the
+// exact tree (non-overlapping pieces, all terms positive).  Masses go to the grids with FP64
+// global atomics.  Precision modes for the clip:
 //   FP64  all FP64;
 //   MIXED FP32 clip on cell-local coordinates formed in FP64, accepted only when its error bound
 //         16 * eps32 * max(1, M)^2 (M = largest |coordinate| of the cell-local tet; twice the worst
 //         ratio measured over 2e5 random tets against FP64) is below 2.5e-7 of the clipped
-//         volume, otherwise recomputed in FP64;
-//   FP32  FP32 clip only (precision study).
+//         volume and it is not noise, otherwise recomputed by the exact FP64 tree;
+//   FP32  FP32 signed clip only (precision study).
+// counters[0] counts the exact recomputes, counters[1] the partial cells.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -67,7 +72,8 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = 64 * kWavesPerBlock;
-constexpr int kBuckets = 9;        // clip work classes
+constexpr int kBuckets = 10;       // 9 clip work classes + the exact-recompute queue
+constexpr int kExactQueue = 9;
 constexpr int kQueueCap = 2 * kWaveTile;
 
 // Per-wavefront LDS: pair-start markers of the current tile and the bucketed partial-cell queues.
@@ -389,8 +395,11 @@ __device__ __forceinline__ void classify_tile(const TileSource& src, int slot, i
     wave_sync();
 }
 
+// Clips one queued partial cell and deposits it, or returns true when the cell must be recomputed
+// by the exact tree (in the exact queue): a signed-tree result that may be cancellation noise, or a
+// mixed-precision FP32 result outside its error bound.
 template <int kPrec, class Sources>
-__device__ __forceinline__ void clip_entry(const Sources& srcs, int2 e, int slot, unsigned long long& fb) {
+__device__ __forceinline__ bool clip_entry(const Sources& srcs, int2 e, int slot, bool exact) {
     const TileSource& src = srcs.get(slot);
     const PairData& pd = src.pd[e.x];
     const unsigned u = unsigned(e.y);
@@ -401,28 +410,31 @@ __device__ __forceinline__ void clip_entry(const Sources& srcs, int2 e, int slot
         V3<double> s[4];
         cell_local(pd, g, i, j, k, s);
         rotate_axes(s, axis);
-        w6 = clip_unit_cube_vol6(s);
+        double sabs;
+        w6 = clip_any_vol6(s, !exact, &sabs);
+        if (!exact && signed_is_noisy(w6, sabs)) return true;
+    } else if (kPrec == ADECO_PREC_MIXED && exact) {
+        V3<double> s[4];
+        cell_local(pd, g, i, j, k, s);
+        rotate_axes(s, axis);
+        w6 = clip_exact_vol6(s);
     } else {
         V3<float> s[4];
         cell_local(pd, g, i, j, k, s);
         rotate_axes(s, axis);
-        const float w32 = clip_unit_cube_vol6(s);
+        float sabs;
+        const float w32 = clip_signed_vol6(s, &sabs);
         w6 = double(w32);
         if (kPrec == ADECO_PREC_MIXED) {
             float M = 1.f;
             for (int q = 0; q < 4; ++q) M = fmaxf(M, fmaxf(fmaxf(fabsf(s[q].x), fabsf(s[q].y)), fabsf(s[q].z)));
             const float bound = 16.f * 5.9604645e-8f * M * M;
-            if (!(bound <= 2.5e-7f * w32)) {
-                V3<double> sd[4];
-                cell_local(pd, g, i, j, k, sd);
-                rotate_axes(sd, axis);
-                w6 = clip_unit_cube_vol6(sd);
-                ++fb;
-            }
+            if (!(bound <= 2.5e-7f * w32) || signed_is_noisy(w32, sabs)) return true;
         }
     }
     const double dm = pd.mcell * (w6 * (1.0 / 6.0));
     if (dm > 0.0) unsafeAtomicAdd(src.grid_buf + g.base + i + int64_t(g.dims) * (j + int64_t(g.dims) * k), dm);
+    return false;
 }
 
 struct OneSource {
@@ -451,12 +463,13 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
     int64_t it = 0, it_end = 0;
     bool draining = false;
     for (;;) {
-        // a queue to clip: full (64) while tiles remain, any non-empty one when draining
+        // a queue to clip: full (64) while tiles remain, any non-empty one when draining; the
+        // exact queue first, so that it holds < 64 whenever another queue's clips requeue into it
         const int need = draining ? 1 : kWaveTile;
         const int qv = lane < kBuckets ? L.qn[lane] : 0;
         const unsigned long long ready = __ballot(qv >= need);
         if (ready) {
-            const int b = __builtin_ctzll(ready);
+            const int b = (ready >> kExactQueue) & 1ull ? kExactQueue : __builtin_ctzll(ready);
             const int cnt = __builtin_amdgcn_readfirstlane(__shfl(qv, b, 64));
             const int take = min(cnt, kWaveTile);
             const int rest = cnt - take;
@@ -469,7 +482,14 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
             wave_sync();
             if (lane == 0) L.qn[b] = rest;
             wave_sync();
-            if (lane < take) clip_entry<kPrec>(srcs, e, slot, fb);
+            const bool requeue = lane < take && clip_entry<kPrec>(srcs, e, slot, b == kExactQueue);
+            if (requeue) {
+                const int pos = atomicAdd(&L.qn[kExactQueue], 1);
+                L.q[kExactQueue][pos] = e;
+                L.qs[kExactQueue][pos] = (unsigned char)slot;
+                ++fb;
+            }
+            wave_sync();
             continue;
         }
         if (draining) break;
@@ -495,7 +515,7 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
         classify_tile(srcs.get(slot), slot, t, L, lane, n_partial);
     }
     if (lane == 0 && n_partial) atomicAdd(counters + 1, n_partial);
-    if (kPrec == ADECO_PREC_MIXED && fb) atomicAdd(counters, fb);
+    if (fb) atomicAdd(counters, fb);
 }
 
 template <int kPrec>
