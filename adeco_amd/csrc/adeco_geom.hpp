@@ -181,6 +181,29 @@ ADECO_HD R beyond_fraction(R d0, R d1, R d2, R d3) {
     return n == 4 ? one : (n == 3 ? one - f : f);
 }
 
+// Both sides of a plane from one closed form: *above = fraction with d > 0, *below = the rest.  The
+// side with fewer vertices is the one computed directly; the other is 1 minus it.
+template <class R>
+ADECO_HD void beyond_split(R d0, R d1, R d2, R d3, R* above, R* below) {
+    const bool b0 = d0 > R(0), b1 = d1 > R(0), b2 = d2 > R(0), b3 = d3 > R(0);
+    const int n = int(b0) + int(b1) + int(b2) + int(b3);
+    const R m0 = absr(d0), m1 = absr(d1), m2 = absr(d2), m3 = absr(d3);
+    const R one = R(1), zero = R(0);
+    const R den = ((b0 != b1) ? m0 + m1 : one) * ((b0 != b2) ? m0 + m2 : one) * ((b0 != b3) ? m0 + m3 : one) *
+                  ((b1 != b2) ? m1 + m2 : one) * ((b1 != b3) ? m1 + m3 : one) * ((b2 != b3) ? m2 + m3 : one);
+    const R p = (b0 ? m0 : one) * (b1 ? m1 : one) * (b2 ? m2 : one) * (b3 ? m3 : one);
+    const R q = (b0 ? one : m0) * (b1 ? one : m1) * (b2 ? one : m2) * (b3 ? one : m3);
+    const R sb = (b0 ? m0 : zero) + (b1 ? m1 : zero) + (b2 ? m2 : zero) + (b3 ? m3 : zero);
+    const R sn = (b0 ? zero : m0) + (b1 ? zero : m1) + (b2 ? zero : m2) + (b3 ? zero : m3);
+    const R num2 = p * p + p * sb * sn + q * (sb * sb - p);
+    const R e = n == 1 ? p : q;
+    const R num = n == 2 ? num2 : e * e * e;
+    const R f = (n == 0 || n == 4) ? zero : qdiv(num, den);
+    // f is the d > 0 side for n = 1, 2 and the d <= 0 side for n = 3
+    *above = n == 4 ? one : (n == 3 ? one - f : f);
+    *below = n == 0 ? one : (n == 3 ? f : (n == 4 ? zero : one - f));
+}
+
 // |vol6| of the part of tet (a,b,c,d) inside the slab 0 <= z <= 1.
 template <class R>
 ADECO_HD R zslab_vol6(const V3<R>& a, const V3<R>& b, const V3<R>& c, const V3<R>& d) {
@@ -336,45 +359,46 @@ struct ClipCounters {
     uint32_t stages, subtets;
 };
 
-// signed z-slab volumes of the pieces of one stage; *sabs accumulates their magnitudes
-template <class R, int NS>
-ADECO_HD R stage_zslab(const Stage<R, NS>& st, R g, R* sabs, ClipCounters* cnt) {
+// Hands the pieces of one stage, with their signs, to the leaf.
+// (unrolled over the at most 3 pieces, so that take() reads fixed slots)
+template <class R, int NS, class Leaf>
+ADECO_HD void stage_leaves(const Stage<R, NS>& st, R g, Leaf& leaf, ClipCounters* cnt) {
     if (cnt) cnt->subtets += uint32_t(st.count);
-    R v = R(0);
-    for (int k = 0; k < st.count; ++k) {
-        V3<R> q[4];
-        const R gk = g * take(st, k, q);
-        const R w = zslab_vol6(q[0], q[1], q[2], q[3]);
-        v += gk * w;
-        *sabs += w;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k < st.count) {
+            V3<R> q[4];
+            const R gk = g * take(st, k, q);
+            leaf(q, gk);
+        }
     }
-    return v;
 }
 
-// The clip tree of t over its cutting x/y faces with the z slab in closed form at the leaves.
-// use_signed: a cut that leaves 3 vertices inside gives the whole tet (+) and the corner beyond the
-// plane (-) instead of 3 prism tets; *sabs returns the sum of the magnitudes of all leaf terms.
-// One instance serves both trees, so a kernel holds the clip code once.
-template <class R, int NS = 7>
-ADECO_HD R clip_tree_vol6(const V3<R> t[4], int act, int nact, bool use_signed, R* sabs, ClipCounters* cnt) {
+// The clip tree of t over its cutting x/y faces; every leaf piece q (sign g) goes to leaf(q, g),
+// which integrates its z extent in closed form.  use_signed: a cut that leaves 3 vertices inside
+// gives the whole tet (+) and the corner beyond the plane (-) instead of 3 prism tets.  One
+// instance serves both trees, so a kernel holds the clip code once.
+template <class R, int NS, class Leaf>
+ADECO_HD void clip_tree(const V3<R> t[4], int act, int nact, bool use_signed, Leaf& leaf, ClipCounters* cnt) {
     int ax;
     R c;
     bool ge;
     Stage<R, NS> s1, s2, s3, s4;
     V3<R> q1[4], q2[4], q3[4];
-    *sabs = R(0);
     face_of(act, 0, ax, c, ge);
     clip_stage(t, ax, c, ge, s1, use_signed);
     if (cnt) cnt->stages += 1;
-    if (nact == 1) return stage_zslab(s1, R(1), sabs, cnt);
-    R vol = R(0);
+    if (nact == 1) {
+        stage_leaves(s1, R(1), leaf, cnt);
+        return;
+    }
     for (int k1 = 0; k1 < s1.count; ++k1) {
         const R g1 = take(s1, k1, q1);
         face_of(act, 1, ax, c, ge);
         clip_stage(q1, ax, c, ge, s2, use_signed);
         if (cnt) cnt->stages += 1;
         if (nact == 2) {
-            vol += stage_zslab(s2, g1, sabs, cnt);
+            stage_leaves(s2, g1, leaf, cnt);
             continue;
         }
         for (int k2 = 0; k2 < s2.count; ++k2) {
@@ -383,7 +407,7 @@ ADECO_HD R clip_tree_vol6(const V3<R> t[4], int act, int nact, bool use_signed, 
             clip_stage(q2, ax, c, ge, s3, use_signed);
             if (cnt) cnt->stages += 1;
             if (nact == 3) {
-                vol += stage_zslab(s3, g2, sabs, cnt);
+                stage_leaves(s3, g2, leaf, cnt);
                 continue;
             }
             for (int k3 = 0; k3 < s3.count; ++k3) {
@@ -391,26 +415,87 @@ ADECO_HD R clip_tree_vol6(const V3<R> t[4], int act, int nact, bool use_signed, 
                 face_of(act, 3, ax, c, ge);
                 clip_stage(q3, ax, c, ge, s4, use_signed);
                 if (cnt) cnt->stages += 1;
-                vol += stage_zslab(s4, g3, sabs, cnt);
+                stage_leaves(s4, g3, leaf, cnt);
             }
         }
     }
-    return vol;
 }
 
-// 6 x |t ∩ [0,1]^3| by the signed (use_signed) or the exact tree; *sabs as above (0 when t needs
-// no clip).  NS = 6 compiles the exact tree alone (use_signed is then ignored).
-template <class R, int NS = 7>
-ADECO_HD R clip_any_vol6(const V3<R> t[4], bool use_signed, R* sabs, ClipCounters* cnt = nullptr) {
-    *sabs = R(0);
+// The pieces of t inside the x/y square [0,1]^2 (any z) to the leaf: t itself when no x/y face
+// cuts it, none when it lies outside one of them.  NS = 6 compiles the exact tree alone
+// (use_signed is then ignored).
+template <class R, int NS = 7, class Leaf>
+ADECO_HD void clip_pieces(const V3<R> t[4], bool use_signed, Leaf& leaf, ClipCounters* cnt = nullptr) {
     const int act = active_xy_faces(t);
-    if (act < 0) return R(0);
+    if (act < 0) return;
     const int nact = (act & 1) + ((act >> 1) & 1) + ((act >> 2) & 1) + ((act >> 3) & 1);
     if (nact == 0) {
         if (cnt) cnt->subtets += 1;
-        return zslab_vol6(t[0], t[1], t[2], t[3]);
+        leaf(t, R(1));
+        return;
     }
-    return clip_tree_vol6<R, NS>(t, act, nact, use_signed, sabs, cnt);
+    clip_tree<R, NS>(t, act, nact, use_signed, leaf, cnt);
+}
+
+// Leaf of the unit cell: signed sum of the pieces' z-slab volumes and the sum of their magnitudes.
+template <class R>
+struct ZslabSum {
+    R vol, sabs;
+    ADECO_HD void operator()(const V3<R> q[4], R g) {
+        const R w = zslab_vol6(q[0], q[1], q[2], q[3]);
+        vol += g * w;
+        sabs += w;
+    }
+};
+
+// Leaf of a column segment of ls unit cells along z ([c, c+1], c < ls; ls = 1 is the unit cell):
+// piece q (sign g, |vol6| w) adds g w (1 - below(c) - above(c+1)) to acc[c * kStride] and
+// w (1 - below(c) - above(c+1)) to sab[c * kStride] (each plane between two cells is evaluated
+// once for both; per cell the same terms as zslab_vol6).
+template <class R, int kStride>
+struct ColumnLeaf {
+    double* acc;
+    double* sab;
+    int ls;
+    ADECO_HD void operator()(const V3<R> q[4], R g) {
+        const R w = absr(vol6(q[0], q[1], q[2], q[3]));
+        const R zmin = fmin(fmin(q[0].z, q[1].z), fmin(q[2].z, q[3].z));
+        const R zmax = fmax(fmax(q[0].z, q[1].z), fmax(q[2].z, q[3].z));
+        const R top = R(ls);
+        // cells [c0, c1] meet the piece's z range; comparisons with NaN are false, which keeps both
+        // in [0, ls)
+        const int c0 = zmin > R(0) ? int(fmin(floor(zmin), top - R(1))) : 0;
+        const int c1 = zmax < top ? int(fmax(ceil(zmax) - R(1), R(0))) : ls - 1;
+        R below_prev = R(0);
+        for (int h = c0; h <= c1 + 1; ++h) {
+            const R zh = R(h);
+            R above = R(0), below = R(1);
+            if (zmin >= zh) {
+                above = R(1);
+                below = R(0);
+            } else if (zmax > zh) {
+                beyond_split(q[0].z - zh, q[1].z - zh, q[2].z - zh, q[3].z - zh, &above, &below);
+            }
+            if (h > c0) {
+                const R m = R(1) - below_prev - above;
+                if (m > R(0)) {
+                    acc[(h - 1) * kStride] += double(g * w * m);
+                    sab[(h - 1) * kStride] += double(w * m);
+                }
+            }
+            below_prev = below;
+        }
+    }
+};
+
+// 6 x |t ∩ [0,1]^3| by the signed (use_signed) or the exact tree; *sabs = the sum of the magnitudes
+// of the terms it was summed from.
+template <class R, int NS = 7>
+ADECO_HD R clip_any_vol6(const V3<R> t[4], bool use_signed, R* sabs, ClipCounters* cnt = nullptr) {
+    ZslabSum<R> acc{R(0), R(0)};
+    clip_pieces<R, NS>(t, use_signed, acc, cnt);
+    *sabs = acc.sabs;
+    return acc.vol;
 }
 
 // Signed tree of a cell-local tet with its noise measure: returns 6 x |t ∩ [0,1]^3| and sets *sabs

@@ -1,39 +1,43 @@
-// Batched persistent deposit: one launch over every (grid, tet, candidate cell) of a candidate space.
+// Batched persistent deposit: one launch over every (grid, tet, candidate) of a candidate space.
 //
 // The plain form gives every lane one tet and lets it walk its covered cells, so a wave runs as
 // long as its largest tet and leaves most lanes idle on heavy-tailed leaves.  Here the unit of
-// work is one (tet, candidate cell) pair, and every wavefront works on its own, with no block
-// barriers:
+// work is one (tet, candidate) pair, and every wavefront works on its own, with no block barriers.
+// A candidate is a column segment: up to kMaxSeg consecutive covered cells along the tet's longest
+// axis (FP64 on grids of unit scaled cells), else a single cell.
 //   1. k_pair_setup     one lane per binned (grid, tet) pair: signed volume and orientation
 //                       (v0 <-> v3 when negative, AdDepositTets2Grid.h:547-558), vertices scaled to
 //                       the grid's cell units in FP64, mass factor, covered cell range
-//                       (AdDepositTets2Grid.h:514-539) -> one 128-byte PairData line and a
-//                       candidate count;
+//                       (AdDepositTets2Grid.h:514-539), column axis -> one 128-byte PairData line
+//                       and a candidate count;
 //   2. scan             candidate counts -> 64-bit offsets, total C;
 //   3. k_tile_map       first pair of every tile of 64 consecutive candidates;
 //   4. k_deposit_waves  persistent wavefronts pop chunks of tiles from a work-queue counter.  Per
 //                       tile each lane finds its candidate's pair (a marker per pair start and a
-//                       wave prefix-max), culls the cell by AABB and classifies it with the tet's
-//                       4 face planes at the extreme cell corners (a relative margin of 1e-12: an
-//                       uncertain cell is clipped, never skipped or filled).  Cells inside the tet
-//                       or holding the whole tet are deposited at once; partial cells go to one of
-//                       9 per-wave LDS queues keyed by their clip work class (number of x/y faces
-//                       that cut the tet once the axis with the most cutting faces is taken as the
-//                       closed-form z axis, and whether the first cut leaves 1 or 3 sub-tets).  A
-//                       queue that reaches 64 entries is clipped by the whole wave, so the lanes of
-//                       one clip follow nearly the same path through the nested clip.
-// The clip uses the signed tree (a cut leaving 3 vertices inside gives the tet minus the corner
-// beyond the plane: fewer pieces); a result below 1e-6 of the magnitudes it was summed from may be
-// cancellation noise, so that cell goes to a tenth per-wave queue whose waves clip it with the
-// exact tree (non-overlapping pieces, all terms positive).  Masses go to the grids with FP64
-// global atomics.  Precision modes for the clip:
+//                       wave prefix-max), culls the segment by AABB and classifies it with the
+//                       tet's 4 face planes at the extreme corners (a relative margin of 1e-12: an
+//                       uncertain segment is clipped, never skipped or filled).  Segments inside the
+//                       tet and cells holding the whole tet are deposited at once; partial ones go
+//                       to one of 9 per-wave LDS queues keyed by their clip work class (number of
+//                       x/y faces that cut the tet once the column axis - for a single cell the axis
+//                       with the most cutting faces - is taken as the closed-form z axis, and
+//                       whether the first cut leaves 1 or 3 sub-tets).  A queue that reaches 64
+//                       entries is clipped by the whole wave, so the lanes of one clip follow nearly
+//                       the same path through the nested clip.
+// One x/y clip serves all cells of a segment: every leaf piece is integrated over the z planes
+// between the cells in closed form, each plane once for the two cells it separates, into per-lane
+// LDS sums.  The clip uses the signed tree (a cut leaving 3 vertices inside gives the tet minus the
+// corner beyond the plane: fewer pieces); a cell whose sum is below 1e-6 of the magnitudes it was
+// summed from may be cancellation noise, so it goes alone to a tenth per-wave queue whose waves
+// clip it with the exact tree (non-overlapping pieces, all terms positive).  Masses go to the grids
+// with FP64 global atomics.  Precision modes for the clip:
 //   FP64  all FP64;
-//   MIXED FP32 clip on cell-local coordinates formed in FP64, accepted only when its error bound
-//         16 * eps32 * max(1, M)^2 (M = largest |coordinate| of the cell-local tet; twice the worst
-//         ratio measured over 2e5 random tets against FP64) is below 2.5e-7 of the clipped
-//         volume and it is not noise, otherwise recomputed by the exact FP64 tree;
+//   MIXED FP32 clip on cell-local coordinates formed in FP64 (single cells), accepted only when its
+//         error bound 16 * eps32 * max(1, M)^2 (M = largest |coordinate| of the cell-local tet;
+//         twice the worst ratio measured over 2e5 random tets against FP64) is below 2.5e-7 of the
+//         clipped volume and it is not noise, otherwise recomputed by the exact FP64 tree;
 //   FP32  FP32 signed clip only (precision study).
-// counters[0] counts the exact recomputes, counters[1] the partial cells.
+// counters[0] counts the exact recomputes (cells), counters[1] the partial candidates.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -53,7 +57,9 @@ GridDesc make_grid_desc(const GridGeom& g, long long base) {
     for (int a = 0; a < 3; ++a) {
         d.gmin[a] = g.gmin[a];
         d.delta[a] = g.delta[a];
-        d.r[a] = g.delta[a] * g.scale[a];
+        // a cell edge of exactly 1 lets neighbouring cells share their z planes (column segments);
+        // it moves cell faces by at most an ulp of the index
+        d.r[a] = g.scale[a] == 1.0 / g.delta[a] ? 1.0 : g.delta[a] * g.scale[a];
         d.scale[a] = g.scale[a];
     }
     d.cell_vol = g.cell_vol;
@@ -76,13 +82,16 @@ constexpr int kBuckets = 10;       // 9 clip work classes + the exact-recompute 
 constexpr int kExactQueue = 9;
 constexpr int kQueueCap = 2 * kWaveTile;
 
-// Per-wavefront LDS: pair-start markers of the current tile and the bucketed partial-cell queues.
-// A queue entry is (pair, cell | closed-form axis << 30) plus the candidate space it belongs to.
+// Per-wavefront LDS: pair-start markers of the current tile, the bucketed partial-candidate queues
+// and each lane's per-cell sums of the column it clips.  A queue entry is (pair, first cell |
+// closed-form axis << 30) plus the candidate space it belongs to.
 struct WaveLds {
     int mark[kWaveTile];
     int qn[16];
     int2 q[kBuckets][kQueueCap];
     unsigned char qs[kBuckets][kQueueCap];
+    double acc[kMaxSeg][kWaveTile];   // signed mass sums (vol6 units) of the column's cells
+    double sab[kMaxSeg][kWaveTile];   // sums of their magnitudes
 };
 
 __device__ __forceinline__ int upper_bound_i64(const int64_t* a, int n, int64_t v) {
@@ -96,10 +105,12 @@ __device__ __forceinline__ int upper_bound_i64(const int64_t* a, int n, int64_t 
 }
 
 // per pair: PairData and candidate count (dead pairs keep one dummy candidate so every pair owns
-// at least one candidate and a tile of 64 candidates spans at most 64 pairs)
+// at least one candidate and a tile of 64 candidates spans at most 64 pairs).  With columns
+// (max_seg > 1 and unit cells) a candidate is a segment of up to max_seg cells along the axis of
+// the largest extent (ties -> z, then y); otherwise a single cell.
 __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorners ct, const int32_t* __restrict__ pairs,
                                                     int64_t n_pairs, const int64_t* __restrict__ grid_off, int n_grids,
-                                                    const GridDesc* __restrict__ grids, double tet_mass,
+                                                    const GridDesc* __restrict__ grids, double tet_mass, int max_seg,
                                                     unsigned* __restrict__ cand, PairData* __restrict__ pd) {
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n_pairs) return;
@@ -118,7 +129,7 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
     unsigned long long pack = 0;
     const double vol = vol6(v[0], v[1], v[2], v[3]) / 6.0;
     const int dm1 = g.dims - 1;
-    unsigned n = 1;
+    unsigned ext[3];
     for (int a = 0; a < 3; ++a) {
         // cells whose box can strictly overlap the tet AABB.  The reference covers [floor, ceil]
         // (AdDepositTets2Grid.h:530-539), one cell more per axis than can pass its strict test
@@ -130,10 +141,17 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
         l = l <= 0 ? 0 : (l >= dm1 ? dm1 : l);
         h = h <= 0 ? 0 : (h >= dm1 ? dm1 : h);
         const unsigned e = unsigned(h - l + 1);
-        n *= e;
+        ext[a] = e;
         pack |= (unsigned long long)l << (10 * a);
         pack |= (unsigned long long)e << (30 + 10 * a);
     }
+    const bool col = max_seg > 1 && g.r[0] == 1.0 && g.r[1] == 1.0 && g.r[2] == 1.0;
+    const int axis = !col ? 2 : (ext[2] >= ext[1] && ext[2] >= ext[0]) ? 2 : (ext[1] >= ext[0] ? 1 : 0);
+    const unsigned ea = axis == 0 ? ext[0] : (axis == 1 ? ext[1] : ext[2]);
+    const unsigned seg = col ? unsigned(max_seg) : 1u;
+    const unsigned n = (ext[0] * ext[1] * ext[2] / ea) * ((ea + seg - 1) / seg);
+    pack |= (unsigned long long)axis << 60;
+    pack |= (unsigned long long)(col ? 1 : 0) << 62;
     const bool live = vol != 0.0;
     if (vol < 0.0) {
         const V3<double> t = v[0];
@@ -266,11 +284,13 @@ __device__ __forceinline__ int classify_margin(const V3<double> s[4], const doub
 // closed form (ties -> z); the class counts the remaining x/y faces that cut (from the cell-local
 // AABB) and whether the first of them leaves one sub-tet or a prism.  The class only groups cells
 // of similar clip work; the clip itself decides every face again.
-__device__ __forceinline__ int work_class(const V3<double> s[4], const double mn[3], const double mx[3], int& axis) {
+// A column segment (force >= 0) keeps its axis: force.
+__device__ __forceinline__ int work_class(const V3<double> s[4], const double mn[3], const double mx[3], int force,
+                                          int& axis) {
     const bool lx = mn[0] < 0.0, hx = mx[0] > 1.0, ly = mn[1] < 0.0, hy = mx[1] > 1.0, lz = mn[2] < 0.0, hz = mx[2] > 1.0;
     const int ax_ = int(lx) + int(hx), ay = int(ly) + int(hy), az = int(lz) + int(hz);
     // closed-form axis: most cutting faces, ties -> z, then x
-    axis = (ax_ > az && ax_ >= ay) ? 0 : ((ay > az && ay > ax_) ? 1 : 2);
+    axis = force >= 0 ? force : (ax_ > az && ax_ >= ay) ? 0 : ((ay > az && ay > ax_) ? 1 : 2);
     // the clip's x and y after rotation: (y, z) for axis 0, (z, x) for axis 1, (x, y) for axis 2
     const bool lo0 = axis == 0 ? ly : (axis == 1 ? lz : lx), hi0 = axis == 0 ? hy : (axis == 1 ? hz : hx);
     const bool lo1 = axis == 0 ? lz : (axis == 1 ? lx : ly), hi1 = axis == 0 ? hz : (axis == 1 ? hx : hy);
@@ -348,14 +368,27 @@ __device__ __forceinline__ void classify_tile(const TileSource& src, int slot, i
         const unsigned long long rg = pd.range;
         const unsigned ex = unsigned(rg >> 30) & 1023u;
         if (ex) {
-            const unsigned ey = unsigned(rg >> 40) & 1023u;
+            const unsigned ey = unsigned(rg >> 40) & 1023u, ez = unsigned(rg >> 50) & 1023u;
+            const int pa = int(rg >> 60) & 3;
+            const bool col = (rg >> 62) & 1ull;
+            // candidate r -> (u fastest, v, segment) over the extents across and along the column
+            // axis pa (u, v = the axes after pa, cyclically)
+            const unsigned eu = pa == 0 ? ey : (pa == 1 ? ez : ex), ev = pa == 0 ? ez : (pa == 1 ? ex : ey);
+            const unsigned ea = pa == 0 ? ex : (pa == 1 ? ey : ez);
+            const unsigned lmax = col ? unsigned(kMaxSeg) : 1u;
             const unsigned r = unsigned(c - off);
-            const unsigned rk = udiv_small(r, ex * ey);
-            const unsigned rij = r - rk * ex * ey;
-            const unsigned rj = udiv_small(rij, ex);
-            const int i = int(rg & 1023u) + int(rij - rj * ex);
-            const int j = int((rg >> 10) & 1023u) + int(rj);
-            const int k = int((rg >> 20) & 1023u) + int(rk);
+            const unsigned rest = udiv_small(r, eu);
+            const unsigned su = r - rest * eu;
+            const unsigned sg = udiv_small(rest, ev);
+            const unsigned sv = rest - sg * ev;
+            const int ls = int(min(lmax, ea - sg * lmax));
+            const int li = int(rg & 1023u), lj = int((rg >> 10) & 1023u), lk = int((rg >> 20) & 1023u);
+            const int cu = (pa == 0 ? lj : (pa == 1 ? lk : li)) + int(su);
+            const int cv = (pa == 0 ? lk : (pa == 1 ? li : lj)) + int(sv);
+            const int ca = (pa == 0 ? li : (pa == 1 ? lj : lk)) + int(sg * lmax);
+            const int i = pa == 0 ? ca : (pa == 1 ? cv : cu);
+            const int j = pa == 0 ? cu : (pa == 1 ? ca : cv);
+            const int k = pa == 0 ? cv : (pa == 1 ? cu : ca);
             const GridDesc& g = src.grids[pd.grid];
             V3<double> s[4];
             cell_local(pd, g, i, j, k, s);
@@ -365,21 +398,27 @@ __device__ __forceinline__ void classify_tile(const TileSource& src, int slot, i
             const double mx[3] = {fmax(fmax(s[0].x, s[1].x), fmax(s[2].x, s[3].x)),
                                   fmax(fmax(s[0].y, s[1].y), fmax(s[2].y, s[3].y)),
                                   fmax(fmax(s[0].z, s[1].z), fmax(s[2].z, s[3].z))};
-            // strict AABB cull against the cell box [0, r]; the reference then classifies and
-            // clips against the unit cube in scaled coordinates (AdDepositTets2Grid.h:587-634)
-            const bool hit = mn[0] < g.r[0] && 0.0 < mx[0] && mn[1] < g.r[1] && 0.0 < mx[1] && mn[2] < g.r[2] &&
-                             0.0 < mx[2];
+            // strict AABB cull against the cell box [0, r] (a segment: ls cells of edge 1 along pa);
+            // the reference then classifies and clips against the unit cube in scaled coordinates
+            // (AdDepositTets2Grid.h:587-634)
+            const double len = double(ls);
+            const double bx = pa == 0 ? len * g.r[0] : g.r[0], by = pa == 1 ? len * g.r[1] : g.r[1],
+                         bz = pa == 2 ? len * g.r[2] : g.r[2];
+            const bool hit = mn[0] < bx && 0.0 < mx[0] && mn[1] < by && 0.0 < mx[1] && mn[2] < bz && 0.0 < mx[2];
             if (hit) {
-                const double unit[3] = {1.0, 1.0, 1.0};
-                const int cls = classify_margin(s, unit);
+                const double box[3] = {pa == 0 ? len : 1.0, pa == 1 ? len : 1.0, pa == 2 ? len : 1.0};
+                const int cls = classify_margin(s, box);
                 const int64_t idx = g.base + i + int64_t(g.dims) * (j + int64_t(g.dims) * k);
-                if (cls == CLS_TET_IN_CELL) {
+                if (cls == CLS_TET_IN_CELL && ls == 1) {
                     unsafeAtomicAdd(src.grid_buf + idx, src.tet_mass);
                 } else if (cls == CLS_CELL_IN_TET) {
-                    unsafeAtomicAdd(src.grid_buf + idx, pd.mcell);
-                } else if (cls == CLS_PARTIAL) {
+                    const int64_t stride = pa == 0 ? 1 : (pa == 1 ? int64_t(g.dims) : int64_t(g.dims) * g.dims);
+                    for (int q = 0; q < ls; ++q) unsafeAtomicAdd(src.grid_buf + idx + q * stride, pd.mcell);
+                } else if (cls != CLS_NONE) {
+                    // a segment keeps the pair's axis (its length is measured along it); a single
+                    // cell takes the axis with the most cutting faces
                     int axis;
-                    key = work_class(s, mn, mx, axis);
+                    key = work_class(s, mn, mx, col && ea > 1 ? pa : -1, axis);
                     cell = int(unsigned(i) | (unsigned(j) << 10) | (unsigned(k) << 20) | (unsigned(axis) << 30));
                 }
             }
@@ -395,46 +434,81 @@ __device__ __forceinline__ void classify_tile(const TileSource& src, int slot, i
     wave_sync();
 }
 
-// Clips one queued partial cell and deposits it, or returns true when the cell must be recomputed
-// by the exact tree (in the exact queue): a signed-tree result that may be cancellation noise, or a
-// mixed-precision FP32 result outside its error bound.
+// Clips one queued partial candidate (a cell, or a column segment from its first cell along its
+// axis) and deposits its cells, except those whose result may be wrong: a signed-tree cell sum that
+// may be cancellation noise, or a mixed-precision FP32 result outside its error bound.  Returns
+// the mask of those cells (bit c = c-th cell along the axis); each is recomputed alone by the
+// exact tree from the exact queue, as a cell entry with axis code 3.  acc/sab: this lane's LDS
+// sums.
 template <int kPrec, class Sources>
-__device__ __forceinline__ bool clip_entry(const Sources& srcs, int2 e, int slot, bool exact) {
+__device__ __forceinline__ unsigned clip_entry(const Sources& srcs, int2 e, int slot, bool exact, double* acc,
+                                               double* sab) {
     const TileSource& src = srcs.get(slot);
     const PairData& pd = src.pd[e.x];
     const unsigned u = unsigned(e.y);
-    const int i = int(u & 1023u), j = int((u >> 10) & 1023u), k = int((u >> 20) & 1023u), axis = int(u >> 30);
+    const int i = int(u & 1023u), j = int((u >> 10) & 1023u), k = int((u >> 20) & 1023u), code = int(u >> 30);
     const GridDesc& g = src.grids[pd.grid];
-    double w6;
-    if (kPrec == ADECO_PREC_FP64) {
+    // segment length: the covered cells from the first one on along the axis, at most kMaxSeg
+    const unsigned long long rg = pd.range;
+    int ls = 1;
+    if (code != 3 && ((rg >> 62) & 1ull)) {
+        const int lo = int((rg >> (10 * code)) & 1023u), ext = int((rg >> (30 + 10 * code)) & 1023u);
+        const int ca = code == 0 ? i : (code == 1 ? j : k);
+        ls = max(1, min(kMaxSeg, lo + ext - ca));
+    }
+    for (int c = 0; c < ls; ++c) {
+        acc[c * kWaveTile] = 0.0;
+        sab[c * kWaveTile] = 0.0;
+    }
+    unsigned redo = 0u;
+    int axis = code;
+    if (kPrec == ADECO_PREC_FP64 || (kPrec == ADECO_PREC_MIXED && exact)) {
         V3<double> s[4];
         cell_local(pd, g, i, j, k, s);
+        if (code == 3) {
+            // a single cell: the axis with the most cutting faces goes to the closed form
+            int cut[3];
+            for (int a = 0; a < 3; ++a) {
+                double lo = comp(s[0], a), hi = lo;
+                for (int q = 1; q < 4; ++q) {
+                    lo = fmin(lo, comp(s[q], a));
+                    hi = fmax(hi, comp(s[q], a));
+                }
+                cut[a] = int(lo < 0.0) + int(hi > 1.0);
+            }
+            axis = (cut[0] > cut[2] && cut[0] >= cut[1]) ? 0 : ((cut[1] > cut[2] && cut[1] > cut[0]) ? 1 : 2);
+        }
         rotate_axes(s, axis);
-        double sabs;
-        w6 = clip_any_vol6(s, !exact, &sabs);
-        if (!exact && signed_is_noisy(w6, sabs)) return true;
-    } else if (kPrec == ADECO_PREC_MIXED && exact) {
-        V3<double> s[4];
-        cell_local(pd, g, i, j, k, s);
-        rotate_axes(s, axis);
-        w6 = clip_exact_vol6(s);
+        ColumnLeaf<double, kWaveTile> leaf{acc, sab, ls};
+        if (kPrec == ADECO_PREC_FP64) clip_pieces<double, 7>(s, !exact, leaf);
+        else clip_pieces<double, 6>(s, false, leaf);
+        if (!exact)
+            for (int c = 0; c < ls; ++c)
+                redo |= unsigned(signed_is_noisy(acc[c * kWaveTile], sab[c * kWaveTile])) << c;
     } else {
         V3<float> s[4];
         cell_local(pd, g, i, j, k, s);
         rotate_axes(s, axis);
-        float sabs;
-        const float w32 = clip_signed_vol6(s, &sabs);
-        w6 = double(w32);
+        ColumnLeaf<float, kWaveTile> leaf{acc, sab, ls};
+        clip_pieces<float, 7>(s, true, leaf);
         if (kPrec == ADECO_PREC_MIXED) {
             float M = 1.f;
             for (int q = 0; q < 4; ++q) M = fmaxf(M, fmaxf(fmaxf(fabsf(s[q].x), fabsf(s[q].y)), fabsf(s[q].z)));
             const float bound = 16.f * 5.9604645e-8f * M * M;
-            if (!(bound <= 2.5e-7f * w32) || signed_is_noisy(w32, sabs)) return true;
+            for (int c = 0; c < ls; ++c) {
+                const float w32 = float(acc[c * kWaveTile]);
+                const bool bad = !(bound <= 2.5e-7f * w32) || signed_is_noisy(w32, float(sab[c * kWaveTile]));
+                redo |= unsigned(bad) << c;
+            }
         }
     }
-    const double dm = pd.mcell * (w6 * (1.0 / 6.0));
-    if (dm > 0.0) unsafeAtomicAdd(src.grid_buf + g.base + i + int64_t(g.dims) * (j + int64_t(g.dims) * k), dm);
-    return false;
+    const int64_t stride = axis == 0 ? 1 : (axis == 1 ? int64_t(g.dims) : int64_t(g.dims) * g.dims);
+    double* cell0 = src.grid_buf + g.base + i + int64_t(g.dims) * (j + int64_t(g.dims) * k);
+    for (int c = 0; c < ls; ++c) {
+        const double dm = pd.mcell * (acc[c * kWaveTile] * (1.0 / 6.0));
+        if (!((redo >> c) & 1u) && dm > 0.0) unsafeAtomicAdd(cell0 + c * stride, dm);
+    }
+    return redo;
 }
 
 struct OneSource {
@@ -462,9 +536,30 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
     unsigned long long n_partial = 0, fb = 0;
     int64_t it = 0, it_end = 0;
     bool draining = false;
+    // cells of the last clip pass left to the exact queue: this lane's entry and the cells' mask
+    unsigned pend = 0u;
+    int2 pe = make_int2(0, 0);
+    int pslot = 0;
     for (;;) {
-        // a queue to clip: full (64) while tiles remain, any non-empty one when draining; the
-        // exact queue first, so that it holds < 64 whenever another queue's clips requeue into it
+        // pending cells go to the exact queue one per lane and round while it holds < 64; at 64 it
+        // is clipped first (exact clips leave no cells behind), so it never holds more than 127
+        if (__ballot(pend != 0u) && L.qn[kExactQueue] < kWaveTile) {
+            if (pend) {
+                const int c = __builtin_ctz(pend);
+                pend &= pend - 1u;
+                const unsigned u = unsigned(pe.y);
+                const int ax = int(u >> 30);
+                const unsigned cell = (u & 0x3fffffffu) + (unsigned(c) << (10 * ax));
+                const int pos = atomicAdd(&L.qn[kExactQueue], 1);
+                L.q[kExactQueue][pos] = make_int2(pe.x, int(cell | (3u << 30)));
+                L.qs[kExactQueue][pos] = (unsigned char)pslot;
+                ++fb;
+            }
+            wave_sync();
+            continue;
+        }
+        // a queue to clip: full (64) while tiles remain, any non-empty one when draining (or when
+        // cells are pending); the exact queue first
         const int need = draining ? 1 : kWaveTile;
         const int qv = lane < kBuckets ? L.qn[lane] : 0;
         const unsigned long long ready = __ballot(qv >= need);
@@ -482,12 +577,15 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
             wave_sync();
             if (lane == 0) L.qn[b] = rest;
             wave_sync();
-            const bool requeue = lane < take && clip_entry<kPrec>(srcs, e, slot, b == kExactQueue);
-            if (requeue) {
-                const int pos = atomicAdd(&L.qn[kExactQueue], 1);
-                L.q[kExactQueue][pos] = e;
-                L.qs[kExactQueue][pos] = (unsigned char)slot;
-                ++fb;
+            // (a pass over another queue starts only with no cells pending; exact clips leave none)
+            if (lane < take) {
+                const unsigned redo =
+                    clip_entry<kPrec>(srcs, e, slot, b == kExactQueue, &L.acc[0][lane], &L.sab[0][lane]);
+                if (redo) {
+                    pend = redo;
+                    pe = e;
+                    pslot = slot;
+                }
             }
             wave_sync();
             continue;
@@ -557,7 +655,8 @@ size_t tile_map_bytes(int64_t n_cand) { return align256(size_t((n_cand + kWaveTi
 
 adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, const PackedCorners& ct,
                                    const int32_t* pairs, int64_t n_pairs, const int64_t* grid_off, int n_grids,
-                                   const GridDesc* d_grids, double tet_mass, void* slab, CandidateSpace& cs) {
+                                   const GridDesc* d_grids, double tet_mass, int precision, void* slab,
+                                   CandidateSpace& cs) {
     char* w = static_cast<char*>(slab);
     const int64_t nblk = (n_pairs + kScanBlock - 1) / kScanBlock;
     cs.pd = reinterpret_cast<PairData*>(w);
@@ -574,7 +673,7 @@ adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, c
     cs.n_tiles = 0;
     if (n_pairs == 0) return ADECO_OK;
     hipLaunchKernelGGL(k_pair_setup, dim3(unsigned((n_pairs + 255) / 256)), dim3(256), 0, stream, lat, ct, pairs, n_pairs,
-                       grid_off, n_grids, d_grids, tet_mass, cs.cand, cs.pd);
+                       grid_off, n_grids, d_grids, tet_mass, precision == ADECO_PREC_FP64 ? kMaxSeg : 1, cs.cand, cs.pd);
     hipLaunchKernelGGL(k_scan_local, dim3(unsigned(nblk)), dim3(256), 0, stream, cs.cand, n_pairs, cs.cand_off, cs.block_sum);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, stream, cs.block_sum, nblk, cs.total);
     hipLaunchKernelGGL(k_scan_add, dim3(unsigned((n_pairs + 1 + 255) / 256)), dim3(256), 0, stream, cs.cand_off, n_pairs,
@@ -635,7 +734,8 @@ hipError_t launch_deposit_step(hipStream_t stream, int precision, const TileSour
 // pair data and scan (work), tile map (tiles); grids are the leaf grids (dims^3 FP64 per leaf, DFS
 // leaf order).  Pair setup, scan and tile map run on `stream`; n_cand is read back.
 adeco_status prepare_leaves_batched(adeco_octree_s* T, hipStream_t stream, int dims, int decomposition, double tet_mass,
-                                    DevBuf& geom, DevBuf& work, DevBuf& tiles, double* grids, TileSource& src) {
+                                    int precision, DevBuf& geom, DevBuf& work, DevBuf& tiles, double* grids,
+                                    TileSource& src) {
     adeco_lattice_s* L = T->L;
     if (dims > 1023) return fail(ADECO_EINVAL, "batched deposit: dims must be < 1024");
     const int nl = int(T->leaves_dfs.size());
@@ -656,7 +756,7 @@ adeco_status prepare_leaves_batched(adeco_octree_s* T, hipStream_t stream, int d
     CandidateSpace cs;
     // (synchronises the stream, so gd may go out of scope afterwards)
     if ((st = build_candidate_space(stream, lv, ct, T->d_pairs.as<int32_t>(), P, T->d_leaf_offset.as<int64_t>(), nl,
-                                    geom.as<GridDesc>(), tet_mass, work.ptr, cs)))
+                                    geom.as<GridDesc>(), tet_mass, precision, work.ptr, cs)))
         return st;
     if ((st = tiles.reserve(tile_map_bytes(cs.n_cand)))) return st;
     if ((st = build_tile_map(stream, cs, tiles.as<int32_t>()))) return st;
@@ -669,7 +769,7 @@ adeco_status deposit_leaves_batched(adeco_octree_s* T, int dims, int decompositi
     adeco_lattice_s* L = T->L;
     DeviceState& ds = *L->ds;
     TileSource src;
-    adeco_status st = prepare_leaves_batched(T, ds.stream, dims, decomposition, tet_mass, T->d_leaf_geom, T->d_work,
+    adeco_status st = prepare_leaves_batched(T, ds.stream, dims, decomposition, tet_mass, precision, T->d_leaf_geom, T->d_work,
                                              T->d_tiles, T->d_grids.as<double>(), src);
     if (st) return st;
     if (src.n_tiles == 0) return ADECO_OK;

@@ -14,13 +14,17 @@ struct adeco_octree_s;
 namespace adeco {
 
 constexpr int kWaveTile = 64;  // candidates per tile == lanes per wavefront
+// Cells per column segment: the unit of work of an FP64 deposit on grids of unit scaled cells is a
+// run of up to kMaxSeg cells along the tet's longest axis (one x/y clip for all of them); other
+// deposits use single cells.
+constexpr int kMaxSeg = 8;
 
 // One output grid: a leaf or a request ROI.  gmin/delta/scale in FP64; base = first cell of the
 // grid in the grid buffer; cells i + dims*(j + dims*k).
 struct GridDesc {
     double gmin[3];
     double delta[3];
-    double r[3];  // delta_a * scale_a: the cell edge in the scaled coordinates
+    double r[3];  // delta_a * scale_a: the cell edge in the scaled coordinates (1 exactly when scale_a = 1/delta_a)
     double scale[3];
     double cell_vol;
     long long base;
@@ -35,14 +39,15 @@ GridDesc make_grid_desc(const GridGeom& g, long long base);
 struct alignas(128) PairData {
     double S[12];              // scale_a * (v_q - gmin_a), vertices in the oriented order
     double mcell;              // tet_mass * cell_vol / |V|
-    unsigned long long range;  // covered cells, 10-bit lo / extent per axis (extent 0: dead pair)
+    unsigned long long range;  // covered cells, 10-bit lo / extent per axis (extent 0: dead pair),
+                               // bits 60-61 the column axis, bit 62 set when candidates are columns
     int grid;
     int pad_[3];
 };
 static_assert(sizeof(PairData) == 128, "PairData is one cache line");
 
-// A candidate space: binned (grid, tet) pairs x covered cells, cut into tiles of kWaveTile
-// consecutive candidates.
+// A candidate space: binned (grid, tet) pairs x their candidates (covered cells, or column segments
+// of covered cells), cut into tiles of kWaveTile consecutive candidates.
 struct TileSource {
     const PairData* pd;
     const int64_t* cand_off;     // candidate offsets, n_pairs + 1
@@ -80,14 +85,17 @@ size_t tile_map_bytes(int64_t n_cand);
 
 // Lays out `cs` in `slab` and runs pair setup + scan (asynchronous; n_cand is read back with a
 // synchronous copy).  Pairs are grid-major: grid_off is the CSR offset of each grid's pairs.
+// precision selects the candidate unit: column segments for ADECO_PREC_FP64 on grids with unit
+// scaled cells, single cells otherwise.
 adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, const PackedCorners& ct,
                                    const int32_t* pairs, int64_t n_pairs, const int64_t* grid_off, int n_grids,
-                                   const GridDesc* d_grids, double tet_mass, void* slab, CandidateSpace& cs);
+                                   const GridDesc* d_grids, double tet_mass, int precision, void* slab,
+                                   CandidateSpace& cs);
 // Writes the tile map into `tiles` (tile_map_bytes(cs.n_cand) bytes).
 adeco_status build_tile_map(hipStream_t stream, CandidateSpace& cs, int32_t* tiles);
 
-// Persistent deposit over one candidate space; counters[0] += FP64 recomputes, counters[1] +=
-// partial cells.  queue_counter must be zero.
+// Persistent deposit over one candidate space (built for the same precision); counters[0] += exact
+// recomputes, counters[1] += partial candidates.  queue_counter must be zero.
 hipError_t launch_deposit_tiles(hipStream_t stream, int precision, const TileSource& src, unsigned* queue_counter,
                                 unsigned long long* counters, int device);
 
@@ -99,7 +107,8 @@ hipError_t launch_deposit_step(hipStream_t stream, int precision, const TileSour
 // Candidate space of an octree's leaf deposit in caller-owned buffers (leaf geometry, pair data +
 // scan, tile map) over the leaf grids `grids`; src describes it for launch_deposit_tiles.
 adeco_status prepare_leaves_batched(adeco_octree_s* T, hipStream_t stream, int dims, int decomposition, double tet_mass,
-                                    DevBuf& geom, DevBuf& work, DevBuf& tiles, double* grids, TileSource& src);
+                                    int precision, DevBuf& geom, DevBuf& work, DevBuf& tiles, double* grids,
+                                    TileSource& src);
 
 // Clears n 32-bit words with a kernel (a memset node in a captured graph is avoided).
 hipError_t launch_zero_u32(unsigned* p, int n, hipStream_t stream);

@@ -66,7 +66,7 @@ adeco_status adeco_deposit_graph_create(adeco_octree T, int dims, int decomposit
     if (e == hipSuccess) e = hipEventCreate(&G->e0);
     if (e == hipSuccess) e = hipEventCreate(&G->e1);
     if (e != hipSuccess) return bail(hip_fail(e, "deposit_graph_create()"));
-    if ((st = prepare_leaves_batched(T, G->stream, dims, decomposition, tet_mass, G->geom, G->work, G->tiles,
+    if ((st = prepare_leaves_batched(T, G->stream, dims, decomposition, tet_mass, precision, G->geom, G->work, G->tiles,
                                      G->grids.as<double>(), G->src)))
         return bail(st);
     unsigned* ctr = G->counters.as<unsigned>();

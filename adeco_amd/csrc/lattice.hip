@@ -79,7 +79,8 @@ adeco_status deposit_soup_batched(DeviceState& ds, const float* d_tets, int64_t 
     LatticeView lv{nullptr, nullptr, nullptr, {0, 0, 0}, d_tets};
     const PackedCorners ct = pack_corners(corner_table(0));
     CandidateSpace cs;
-    if ((st = build_candidate_space(ds.stream, lv, ct, ids.as<int32_t>(), n_tets, d_goff, 1, d_gd, tet_mass, work.ptr, cs)))
+    if ((st = build_candidate_space(ds.stream, lv, ct, ids.as<int32_t>(), n_tets, d_goff, 1, d_gd, tet_mass, precision,
+                                    work.ptr, cs)))
         return st;
     if ((st = tiles.reserve(tile_map_bytes(cs.n_cand))) || (st = build_tile_map(ds.stream, cs, tiles.as<int32_t>()))) return st;
     unsigned* q = ctr.as<unsigned>();

@@ -367,7 +367,7 @@ adeco_status adeco_server_s::admit(Req* r, int si) {
     if (r->n_tets > 0) {
         CandidateSpace cs;
         if ((st = build_candidate_space(stream, lv, ct, s.pairs.as<int32_t>(), r->n_tets, d_goff, 1, d_gd, cfg.tet_mass,
-                                        s.work.ptr, cs)))
+                                        cfg.precision, s.work.ptr, cs)))
             return st;
         if ((st = s.tiles.reserve(tile_map_bytes(cs.n_cand)))) return st;
         if ((st = build_tile_map(stream, cs, s.tiles.as<int32_t>()))) return st;

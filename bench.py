@@ -408,7 +408,8 @@ def main():
         total = float(grids64.sum())
         drift = abs(total - N ** 3) / N ** 3
         log(f"[bench] batched fp64: {step_ms:.2f} ms/step, kernel {main_ms:.2f} ms, {value:.3e} tets/s, "
-            f"candidates {tl['candidates']}, partials {tl['partials']}, mass drift {drift:.2e}")
+            f"candidates {tl['candidates']}, partials {tl['partials']}, exact recomputes {tl['fallbacks']}, "
+            f"mass drift {drift:.2e}")
 
         extra = {}
         # the same deposit captured as a hipGraph and replayed (candidate space built once)
