@@ -450,8 +450,14 @@ struct ZslabSum {
 
 // Leaf of a column segment of ls unit cells along z ([c, c+1], c < ls; ls = 1 is the unit cell):
 // piece q (sign g, |vol6| w) adds g w (1 - below(c) - above(c+1)) to acc[c * kStride] and
-// w (1 - below(c) - above(c+1)) to sab[c * kStride] (each plane between two cells is evaluated
-// once for both; per cell the same terms as zslab_vol6).
+// w (1 - below(c) - above(c+1)) to sab[c * kStride].  The fractions depend on the 4 heights only:
+// sorted once (z0 <= z1 <= z2 <= z3), a plane h in (z0, z3) has 1, 2 or 3 heights below it, and
+// the side with fewer vertices is computed directly as in beyond_fraction (D is fixed per interval):
+//   (z0, z1]: below = a^3 / D1                 a = h - z0,  D1 = (z1-z0)(z2-z0)(z3-z0)
+//   (z1, z2]: below = [(ab)^2 + ab(a+b)(c+e) + ce((a+b)^2 - ab)] / D2
+//             a, b = h - z0, h - z1;  c, e = z2 - h, z3 - h;  D2 = (z2-z0)(z3-z0)(z2-z1)(z3-z1)
+//   (z2, z3): above = e^3 / D3                 D3 = (z3-z0)(z3-z1)(z3-z2)
+// Each plane between two cells is evaluated once for both.
 template <class R, int kStride>
 struct ColumnLeaf {
     double* acc;
@@ -459,22 +465,38 @@ struct ColumnLeaf {
     int ls;
     ADECO_HD void operator()(const V3<R> q[4], R g) {
         const R w = absr(vol6(q[0], q[1], q[2], q[3]));
-        const R zmin = fmin(fmin(q[0].z, q[1].z), fmin(q[2].z, q[3].z));
-        const R zmax = fmax(fmax(q[0].z, q[1].z), fmax(q[2].z, q[3].z));
+        R z0 = q[0].z, z1 = q[1].z, z2 = q[2].z, z3 = q[3].z;
+#define ADECO_MINMAX(u, v)      \
+    {                           \
+        const R lo_ = fmin(u, v); \
+        v = fmax(u, v);         \
+        u = lo_;                \
+    }
+        ADECO_MINMAX(z0, z1) ADECO_MINMAX(z2, z3) ADECO_MINMAX(z0, z2) ADECO_MINMAX(z1, z3) ADECO_MINMAX(z1, z2)
+#undef ADECO_MINMAX
         const R top = R(ls);
         // cells [c0, c1] meet the piece's z range; comparisons with NaN are false, which keeps both
         // in [0, ls)
-        const int c0 = zmin > R(0) ? int(fmin(floor(zmin), top - R(1))) : 0;
-        const int c1 = zmax < top ? int(fmax(ceil(zmax) - R(1), R(0))) : ls - 1;
+        const int c0 = z0 > R(0) ? int(fmin(floor(z0), top - R(1))) : 0;
+        const int c1 = z3 < top ? int(fmax(ceil(z3) - R(1), R(0))) : ls - 1;
+        const R d20 = z2 - z0, d30 = z3 - z0, d31 = z3 - z1;
+        const R D1 = (z1 - z0) * d20 * d30, D2 = d20 * d30 * (z2 - z1) * d31, D3 = d30 * d31 * (z3 - z2);
         R below_prev = R(0);
         for (int h = c0; h <= c1 + 1; ++h) {
             const R zh = R(h);
             R above = R(0), below = R(1);
-            if (zmin >= zh) {
+            if (z0 >= zh) {
                 above = R(1);
                 below = R(0);
-            } else if (zmax > zh) {
-                beyond_split(q[0].z - zh, q[1].z - zh, q[2].z - zh, q[3].z - zh, &above, &below);
+            } else if (z3 > zh) {
+                const R a = zh - z0, b = zh - z1, c = z2 - zh, e = z3 - zh;
+                const bool one = zh <= z1, three = zh > z2;
+                const R ab = a * b, apb = a + b;
+                const R num2 = ab * ab + ab * apb * (c + e) + c * e * (apb * apb - ab);
+                const R num = one ? a * a * a : (three ? e * e * e : num2);
+                const R f = qdiv(num, one ? D1 : (three ? D3 : D2));
+                below = three ? R(1) - f : f;
+                above = three ? f : R(1) - f;
             }
             if (h > c0) {
                 const R m = R(1) - below_prev - above;
