@@ -568,6 +568,39 @@ ADECO_HD V3<R> cross(const V3<R>& a, const V3<R>& b) {
 template <class R>
 ADECO_HD V3<R> sub(const V3<R>& a, const V3<R>& b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
 
+// Conservative classification of the positively oriented cell-local tet s against the box [0,r]
+// (FP64): per tet face, the box corners' plane values at their extremes (from the normal's
+// signs), with a relative margin of 1e-12 so that an uncertain box is PARTIAL, never NONE or
+// CELL_IN_TET.
+ADECO_HD int classify_margin(const V3<double> s[4], const double* r) {
+    const V3<double> e01 = sub(s[1], s[0]), e02 = sub(s[2], s[0]), e03 = sub(s[3], s[0]);
+    const V3<double> e13 = sub(s[3], s[1]), e12 = sub(s[2], s[1]);
+    const V3<double> n[4] = {cross(e01, e02), cross(e02, e03), cross(e03, e01), cross(e13, e12)};
+    bool all_in = true;
+    for (int f = 0; f < 4; ++f) {
+        const V3<double> o = f == 3 ? s[1] : s[0];
+        const double v0 = o.x * n[f].x + o.y * n[f].y + o.z * n[f].z;
+        // plane value (o - p).n at the cube corners p in {0, r}: extremes from the normal's signs
+        const double px = n[f].x * r[0], py = n[f].y * r[1], pz = n[f].z * r[2];
+        const double pos = fmax(px, 0.0) + fmax(py, 0.0) + fmax(pz, 0.0);
+        const double neg = fmin(px, 0.0) + fmin(py, 0.0) + fmin(pz, 0.0);
+        const double scl = (fabs(n[f].x) + fabs(n[f].y) + fabs(n[f].z)) *
+                           (1.0 + fmax(fmax(fabs(o.x), fabs(o.y)), fabs(o.z)));
+        const double tol = 1e-12 * scl;
+        const double vmax = v0 - neg;
+        const double vmin = v0 - pos;
+        if (vmax < -tol) return CLS_NONE;
+        all_in = all_in && (vmin > tol);
+    }
+    if (all_in) return CLS_CELL_IN_TET;
+    bool tin = true;
+    for (int q = 0; q < 4; ++q)
+        tin = tin && s[q].x >= 0.0 && s[q].x <= r[0] && s[q].y >= 0.0 && s[q].y <= r[1] && s[q].z >= 0.0 &&
+              s[q].z <= r[2];
+    return tin ? CLS_TET_IN_CELL : CLS_PARTIAL;
+}
+
+// The reference's corner test against the unit cell (described above the CellClass values).
 template <class R>
 ADECO_HD int classify_unit(const V3<R> s[4]) {
     const V3<R> e01 = sub(s[1], s[0]), e02 = sub(s[2], s[0]), e03 = sub(s[3], s[0]);

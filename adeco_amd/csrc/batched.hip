@@ -251,33 +251,25 @@ __device__ __forceinline__ void rotate_axes(V3<R> s[4], int axis) {
     }
 }
 
-// Conservative classification of the cell-local tet s against [0,r] (FP64).
-__device__ __forceinline__ int classify_margin(const V3<double> s[4], const double* r) {
-    const V3<double> e01 = sub(s[1], s[0]), e02 = sub(s[2], s[0]), e03 = sub(s[3], s[0]);
-    const V3<double> e13 = sub(s[3], s[1]), e12 = sub(s[2], s[1]);
-    const V3<double> n[4] = {cross(e01, e02), cross(e02, e03), cross(e03, e01), cross(e13, e12)};
-    bool all_in = true;
+// Cells [c, c+1] of a column segment (in the rotated frame: x, y in [0,1], c < ls) that one face
+// plane of the positively oriented tet t separates from it: bit c.  The face-plane part of
+// classify_margin per cell; the plane offset moves by c * n.z from cell to cell, and the margin
+// takes the largest coordinate over the segment.
+__device__ __forceinline__ unsigned separated_cells(const V3<double> t[4], int ls) {
+    const V3<double> e01 = sub(t[1], t[0]), e02 = sub(t[2], t[0]), e03 = sub(t[3], t[0]);
+    const V3<double> e13 = sub(t[3], t[1]), e12 = sub(t[2], t[1]);
+    unsigned sep = 0u;
     for (int f = 0; f < 4; ++f) {
-        const V3<double> o = f == 3 ? s[1] : s[0];
-        const double v0 = o.x * n[f].x + o.y * n[f].y + o.z * n[f].z;
-        // plane value (o - p).n at the cube corners p in {0, r}: extremes from the normal's signs
-        const double px = n[f].x * r[0], py = n[f].y * r[1], pz = n[f].z * r[2];
-        const double pos = fmax(px, 0.0) + fmax(py, 0.0) + fmax(pz, 0.0);
-        const double neg = fmin(px, 0.0) + fmin(py, 0.0) + fmin(pz, 0.0);
-        const double scl = (fabs(n[f].x) + fabs(n[f].y) + fabs(n[f].z)) *
-                           (1.0 + fmax(fmax(fabs(o.x), fabs(o.y)), fabs(o.z)));
-        const double tol = 1e-12 * scl;
-        const double vmax = v0 - neg;
-        const double vmin = v0 - pos;
-        if (vmax < -tol) return CLS_NONE;
-        all_in = all_in && (vmin > tol);
+        const V3<double> n = f == 0 ? cross(e01, e02) : (f == 1 ? cross(e02, e03) : (f == 2 ? cross(e03, e01) : cross(e13, e12)));
+        const V3<double> o = f == 3 ? t[1] : t[0];
+        const double v0 = o.x * n.x + o.y * n.y + o.z * n.z;
+        // the cell corner p with the largest plane value (o - p).n: p.x, p.y in {0, 1}, p.z = c or c + 1
+        const double neg = fmin(n.x, 0.0) + fmin(n.y, 0.0) + fmin(n.z, 0.0);
+        const double tol = 1e-12 * (fabs(n.x) + fabs(n.y) + fabs(n.z)) *
+                           (1.0 + fmax(fmax(fabs(o.x), fabs(o.y)), fabs(o.z) + double(ls)));
+        for (int c = 0; c < ls; ++c) sep |= unsigned(v0 - double(c) * n.z - neg < -tol) << c;
     }
-    if (all_in) return CLS_CELL_IN_TET;
-    bool tin = true;
-    for (int q = 0; q < 4; ++q)
-        tin = tin && s[q].x >= 0.0 && s[q].x <= r[0] && s[q].y >= 0.0 && s[q].y <= r[1] && s[q].z >= 0.0 &&
-              s[q].z <= r[2];
-    return tin ? CLS_TET_IN_CELL : CLS_PARTIAL;
+    return sep;
 }
 
 // Clip work class of a partial cell: the axis with the most faces cutting the tet is integrated in
@@ -438,8 +430,8 @@ __device__ __forceinline__ void classify_tile(const TileSource& src, int slot, i
 // axis) and deposits its cells, except those whose result may be wrong: a signed-tree cell sum that
 // may be cancellation noise, or a mixed-precision FP32 result outside its error bound.  Returns
 // the mask of those cells (bit c = c-th cell along the axis); each is recomputed alone by the
-// exact tree from the exact queue, as a cell entry with axis code 3.  acc/sab: this lane's LDS
-// sums.
+// exact tree from the exact queue, as a cell entry with axis code 3, unless the face-plane test
+// shows that the tet misses it.  acc/sab: this lane's LDS sums.
 template <int kPrec, class Sources>
 __device__ __forceinline__ unsigned clip_entry(const Sources& srcs, int2 e, int slot, bool exact, double* acc,
                                                double* sab) {
@@ -460,7 +452,7 @@ __device__ __forceinline__ unsigned clip_entry(const Sources& srcs, int2 e, int 
         acc[c * kWaveTile] = 0.0;
         sab[c * kWaveTile] = 0.0;
     }
-    unsigned redo = 0u;
+    unsigned redo = 0u, empty = 0u;
     int axis = code;
     if (kPrec == ADECO_PREC_FP64 || (kPrec == ADECO_PREC_MIXED && exact)) {
         V3<double> s[4];
@@ -479,12 +471,17 @@ __device__ __forceinline__ unsigned clip_entry(const Sources& srcs, int2 e, int 
             axis = (cut[0] > cut[2] && cut[0] >= cut[1]) ? 0 : ((cut[1] > cut[2] && cut[1] > cut[0]) ? 1 : 2);
         }
         rotate_axes(s, axis);
+        // most cells whose signed sum is noise lie outside the tet (pieces that cancel there): a
+        // cell that a face plane of the tet separates from it holds nothing
+        const unsigned sep = exact ? 0u : separated_cells(s, ls);
         ColumnLeaf<double, kWaveTile> leaf{acc, sab, ls};
         if (kPrec == ADECO_PREC_FP64) clip_pieces<double, 7>(s, !exact, leaf);
         else clip_pieces<double, 6>(s, false, leaf);
         if (!exact)
             for (int c = 0; c < ls; ++c)
                 redo |= unsigned(signed_is_noisy(acc[c * kWaveTile], sab[c * kWaveTile])) << c;
+        empty = redo & sep;
+        redo &= ~empty;
     } else {
         V3<float> s[4];
         cell_local(pd, g, i, j, k, s);
@@ -506,7 +503,7 @@ __device__ __forceinline__ unsigned clip_entry(const Sources& srcs, int2 e, int 
     double* cell0 = src.grid_buf + g.base + i + int64_t(g.dims) * (j + int64_t(g.dims) * k);
     for (int c = 0; c < ls; ++c) {
         const double dm = pd.mcell * (acc[c * kWaveTile] * (1.0 / 6.0));
-        if (!((redo >> c) & 1u) && dm > 0.0) unsafeAtomicAdd(cell0 + c * stride, dm);
+        if (!(((redo | empty) >> c) & 1u) && dm > 0.0) unsafeAtomicAdd(cell0 + c * stride, dm);
     }
     return redo;
 }
