@@ -341,20 +341,6 @@ ADECO_HD int active_xy_faces(const V3<R> t[4]) {
     return act;
 }
 
-template <class R>
-ADECO_HD void face_of(int act, int level, int& axis, R& c, bool& ge) {
-    // level-th set bit of act
-    int f = 0, seen = 0;
-    for (int i = 0; i < 4; ++i)
-        if (act & (1 << i)) {
-            if (seen == level) f = i;
-            ++seen;
-        }
-    axis = f >> 1;
-    ge = (f & 1) == 0;
-    c = ge ? R(0) : R(1);
-}
-
 struct ClipCounters {
     uint32_t stages, subtets;
 };
@@ -378,24 +364,24 @@ ADECO_HD void stage_leaves(const Stage<R, NS>& st, R g, Leaf& leaf, ClipCounters
 // which integrates its z extent in closed form.  use_signed: a cut that leaves 3 vertices inside
 // gives the whole tet (+) and the corner beyond the plane (-) instead of 3 prism tets.  One
 // instance serves both trees, so a kernel holds the clip code once.
+// t is in the canonical frame of clip_pieces: the cutting faces are, by level, x >= 0, then x <= 1
+// (two x faces) or y >= 0 (one x face: `xy`), then y >= 0, then y <= 1, so only the second level's
+// face is chosen at run time.
 template <class R, int NS, class Leaf>
-ADECO_HD void clip_tree(const V3<R> t[4], int act, int nact, bool use_signed, Leaf& leaf, ClipCounters* cnt) {
-    int ax;
-    R c;
-    bool ge;
+ADECO_HD void clip_tree(const V3<R> t[4], int nact, bool xy, bool use_signed, Leaf& leaf, ClipCounters* cnt) {
     Stage<R, NS> s1, s2, s3, s4;
     V3<R> q1[4], q2[4], q3[4];
-    face_of(act, 0, ax, c, ge);
-    clip_stage(t, ax, c, ge, s1, use_signed);
+    clip_stage(t, 0, R(0), true, s1, use_signed);
     if (cnt) cnt->stages += 1;
     if (nact == 1) {
         stage_leaves(s1, R(1), leaf, cnt);
         return;
     }
+    const int ax1 = xy ? 1 : 0;
+    const R c1 = xy ? R(0) : R(1);
     for (int k1 = 0; k1 < s1.count; ++k1) {
         const R g1 = take(s1, k1, q1);
-        face_of(act, 1, ax, c, ge);
-        clip_stage(q1, ax, c, ge, s2, use_signed);
+        clip_stage(q1, ax1, c1, xy, s2, use_signed);
         if (cnt) cnt->stages += 1;
         if (nact == 2) {
             stage_leaves(s2, g1, leaf, cnt);
@@ -403,8 +389,7 @@ ADECO_HD void clip_tree(const V3<R> t[4], int act, int nact, bool use_signed, Le
         }
         for (int k2 = 0; k2 < s2.count; ++k2) {
             const R g2 = g1 * take(s2, k2, q2);
-            face_of(act, 2, ax, c, ge);
-            clip_stage(q2, ax, c, ge, s3, use_signed);
+            clip_stage(q2, 1, R(0), true, s3, use_signed);
             if (cnt) cnt->stages += 1;
             if (nact == 3) {
                 stage_leaves(s3, g2, leaf, cnt);
@@ -412,8 +397,7 @@ ADECO_HD void clip_tree(const V3<R> t[4], int act, int nact, bool use_signed, Le
             }
             for (int k3 = 0; k3 < s3.count; ++k3) {
                 const R g3 = g2 * take(s3, k3, q3);
-                face_of(act, 3, ax, c, ge);
-                clip_stage(q3, ax, c, ge, s4, use_signed);
+                clip_stage(q3, 1, R(1), false, s4, use_signed);
                 if (cnt) cnt->stages += 1;
                 stage_leaves(s4, g3, leaf, cnt);
             }
@@ -423,18 +407,29 @@ ADECO_HD void clip_tree(const V3<R> t[4], int act, int nact, bool use_signed, Le
 
 // The pieces of t inside the x/y square [0,1]^2 (any z) to the leaf: t itself when no x/y face
 // cuts it, none when it lies outside one of them.  NS = 6 compiles the exact tree alone
-// (use_signed is then ignored).
+// (use_signed is then ignored).  The tree runs in a canonical frame: x and y swapped when y has
+// more cutting faces, and an axis with only its face at 1 cutting mirrored (u -> 1 - u), so that
+// its cutting face becomes the one at 0; volumes are unchanged.
 template <class R, int NS = 7, class Leaf>
 ADECO_HD void clip_pieces(const V3<R> t[4], bool use_signed, Leaf& leaf, ClipCounters* cnt = nullptr) {
     const int act = active_xy_faces(t);
     if (act < 0) return;
-    const int nact = (act & 1) + ((act >> 1) & 1) + ((act >> 2) & 1) + ((act >> 3) & 1);
+    const int nx = (act & 1) + ((act >> 1) & 1), ny = ((act >> 2) & 1) + ((act >> 3) & 1);
+    const int nact = nx + ny;
     if (nact == 0) {
         if (cnt) cnt->subtets += 1;
         leaf(t, R(1));
         return;
     }
-    clip_tree<R, NS>(t, act, nact, use_signed, leaf, cnt);
+    const bool swap = ny > nx;
+    const int ax_ = swap ? (act >> 2) : act, ay_ = swap ? act : (act >> 2);  // bit 0: face at 0, bit 1: face at 1
+    const bool mx = (ax_ & 3) == 2, my = (ay_ & 3) == 2;
+    V3<R> c[4];
+    for (int q = 0; q < 4; ++q) {
+        const R u = swap ? t[q].y : t[q].x, v = swap ? t[q].x : t[q].y;
+        c[q] = mk(mx ? R(1) - u : u, my ? R(1) - v : v, t[q].z);
+    }
+    clip_tree<R, NS>(c, nact, nx == 1 && ny == 1, use_signed, leaf, cnt);
 }
 
 // Leaf of the unit cell: signed sum of the pieces' z-slab volumes and the sum of their magnitudes.

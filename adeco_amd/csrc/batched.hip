@@ -108,13 +108,15 @@ __device__ __forceinline__ int upper_bound_i64(const int64_t* a, int n, int64_t 
 // at least one candidate and a tile of 64 candidates spans at most 64 pairs).  With columns
 // (max_seg > 1 and unit cells) a candidate is a segment of up to max_seg cells along the axis of
 // the largest extent (ties -> z, then y); otherwise a single cell.
+// (pair_grid: the grid of every pair, or null to search grid_off)
 __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorners ct, const int32_t* __restrict__ pairs,
                                                     int64_t n_pairs, const int64_t* __restrict__ grid_off, int n_grids,
+                                                    const int32_t* __restrict__ pair_grid,
                                                     const GridDesc* __restrict__ grids, double tet_mass, int max_seg,
                                                     unsigned* __restrict__ cand, PairData* __restrict__ pd) {
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n_pairs) return;
-    const int gi = upper_bound_i64(grid_off, n_grids + 1, p) - 1;
+    const int gi = pair_grid ? pair_grid[p] : upper_bound_i64(grid_off, n_grids + 1, p) - 1;
     V3<double> v[4];
     lattice_tet_vertices(lat, ct, pairs[p], v);
     const GridDesc& g = grids[gi];
@@ -169,6 +171,13 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
     o.grid = gi;
     o.pad_[0] = o.pad_[1] = o.pad_[2] = 0;
     cand[p] = live ? n : 1u;
+}
+
+// grid index of every pair of grid-major pairs: one workgroup per grid writes its range
+__global__ void __launch_bounds__(256) k_pair_grid(const int64_t* __restrict__ grid_off, int32_t* __restrict__ pair_grid) {
+    const int g = int(blockIdx.x);
+    const int64_t end = grid_off[g + 1];
+    for (int64_t i = grid_off[g] + threadIdx.x; i < end; i += blockDim.x) pair_grid[i] = g;
 }
 
 // exclusive scan of uint32 counts into int64 offsets: block-local pass, block-sum pass, add pass
@@ -645,7 +654,7 @@ int resident_blocks(int device) {
 size_t candidate_space_bytes(int64_t n_pairs) {
     const int64_t nblk = (n_pairs + kScanBlock - 1) / kScanBlock;
     return align256(size_t(n_pairs) * sizeof(PairData)) + align256(n_pairs * 4) + align256((n_pairs + 1) * 8) +
-           align256(nblk * 8 + 8) + align256(8);
+           align256(nblk * 8 + 8) + align256(8) + align256(n_pairs * 4);
 }
 
 size_t tile_map_bytes(int64_t n_cand) { return align256(size_t((n_cand + kWaveTile - 1) / kWaveTile + 1) * 4); }
@@ -665,12 +674,15 @@ adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, c
     cs.block_sum = reinterpret_cast<int64_t*>(w);
     w += align256(nblk * 8 + 8);
     cs.total = reinterpret_cast<int64_t*>(w);
+    w += align256(8);
+    int32_t* pair_grid = n_grids > 1 ? reinterpret_cast<int32_t*>(w) : nullptr;
     cs.n_pairs = n_pairs;
     cs.n_cand = 0;
     cs.n_tiles = 0;
     if (n_pairs == 0) return ADECO_OK;
+    if (pair_grid) hipLaunchKernelGGL(k_pair_grid, dim3(unsigned(n_grids)), dim3(256), 0, stream, grid_off, pair_grid);
     hipLaunchKernelGGL(k_pair_setup, dim3(unsigned((n_pairs + 255) / 256)), dim3(256), 0, stream, lat, ct, pairs, n_pairs,
-                       grid_off, n_grids, d_grids, tet_mass, precision == ADECO_PREC_FP64 ? kMaxSeg : 1, cs.cand, cs.pd);
+                       grid_off, n_grids, pair_grid, d_grids, tet_mass, precision == ADECO_PREC_FP64 ? kMaxSeg : 1, cs.cand, cs.pd);
     hipLaunchKernelGGL(k_scan_local, dim3(unsigned(nblk)), dim3(256), 0, stream, cs.cand, n_pairs, cs.cand_off, cs.block_sum);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, stream, cs.block_sum, nblk, cs.total);
     hipLaunchKernelGGL(k_scan_add, dim3(unsigned((n_pairs + 1 + 255) / 256)), dim3(256), 0, stream, cs.cand_off, n_pairs,
