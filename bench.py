@@ -43,7 +43,7 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (the roof of an FP64 VALU-bou
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of k_deposit_waves<FP64> on this workload: rocprofv3 FETCH_SIZE x 2 +
 # WRITE_SIZE x 1 (each scaled by what it reads for a 1 GiB copy), profiles/deposit_waves_c2_pmc.md
-TRAFFIC_BYTES = 4.809e9
+TRAFFIC_BYTES = 5.526e9
 
 
 def log(*a):
@@ -352,6 +352,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the plain and mixed comparison runs")
+    ap.add_argument("--shape", default="", help="launch shape CHUNK,WORKGROUPS_PER_CU instead of the autotuner's")
     ap.add_argument("--serve", action="store_true", help="run only the ROI serving study (256^3 lattice)")
     ap.add_argument("--precision", action="store_true", help="run only the precision-mode study")
     ap.add_argument("--heavy-tail", action="store_true", help="run only the 256^3 heavy-tailed batched study")
@@ -397,10 +398,17 @@ def main():
         log(f"[bench] octree {tree.num_nodes} nodes / {tree.num_leaves} leaves in {t_oct*1e3:.1f} ms; "
             f"binning {counts.sum()} pairs in {t_bin*1e3:.1f} ms")
 
-        # launch shape by the autotuner (untimed), then the headline: batched persistent form, FP64
-        t0 = time.perf_counter()
-        shape = tree.autotune(DIMS, decomp, TET_MASS, A.PREC_FP64, reps=1)
-        log(f"[bench] autotune: {shape} in {time.perf_counter() - t0:.1f}s")
+        # launch shape by the autotuner (untimed) or --shape, then the headline: batched persistent
+        # form, FP64
+        if args.shape:
+            chunk, blocks = (int(v) for v in args.shape.split(","))
+            A.set_launch_shape(chunk, blocks)
+            shape = {"chunk": chunk, "blocks_per_cu": blocks}
+            log(f"[bench] launch shape {shape}")
+        else:
+            t0 = time.perf_counter()
+            shape = tree.autotune(DIMS, decomp, TET_MASS, A.PREC_FP64, reps=1)
+            log(f"[bench] autotune: {shape} in {time.perf_counter() - t0:.1f}s")
         wall, main_ms, tl = run_steps(tree, args.steps, args.warmup, A.MODE_BATCHED, A.PREC_FP64, decomp)
         step_ms = wall / args.steps * 1e3
         value = lat.num_tets / (wall / args.steps)
