@@ -344,12 +344,40 @@ __device__ __forceinline__ int64_t shfl_i64(int64_t v, int src) {
 
 // Classifies the 64 candidates of tile t of `src` (slot `slot` among the launch's sources):
 // trivial cells are deposited, partial cells are appended to the wave's queues.
-__device__ __forceinline__ void classify_tile(const TileSource& src, int slot, int64_t t, WaveLds& L, int lane,
+// What a tile's classification needs first: its source slot and index, its first pair, the number
+// of pairs it touches and, per lane, the candidate offset of pair p0 + lane.  Loaded one tile
+// ahead, so that these dependent loads overlap the previous tile's work.
+struct TileHead {
+    int slot;
+    int p0, np;
+    int64_t t;
+    int64_t my_off;
+};
+
+template <class Sources>
+__device__ __forceinline__ TileHead load_head(const Sources& srcs, const int2* __restrict__ entries, int64_t item,
+                                              int lane) {
+    TileHead h;
+    h.slot = 0;
+    h.t = item;
+    if (entries) {
+        const int2 en = entries[item];
+        h.slot = __builtin_amdgcn_readfirstlane(en.x);
+        h.t = int64_t(__builtin_amdgcn_readfirstlane(en.y));
+    }
+    const TileSource& src = srcs.get(h.slot);
+    h.p0 = src.tile_first[h.t];
+    h.np = min(src.tile_first[h.t + 1] - h.p0 + 1, kWaveTile);
+    h.my_off = lane < h.np ? src.cand_off[h.p0 + lane] : int64_t(0);
+    return h;
+}
+
+__device__ __forceinline__ void classify_tile(const TileSource& src, const TileHead& h, WaveLds& L, int lane,
                                               unsigned long long& n_partial) {
-    const int p0 = src.tile_first[t];
-    const int np = min(src.tile_first[t + 1] - p0 + 1, kWaveTile);
+    const int slot = h.slot, p0 = h.p0, np = h.np;
+    const int64_t t = h.t;
     const int64_t c0 = t * kWaveTile;
-    const int64_t my_off = lane < np ? src.cand_off[p0 + lane] : int64_t(0);
+    const int64_t my_off = h.my_off;
     L.mark[lane] = 0;
     wave_sync();
     // every pair owns >= 1 candidate, so pairs p0+1.. start at distinct positions 1..63
@@ -541,6 +569,9 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
     wave_sync();
     unsigned long long n_partial = 0, fb = 0;
     int64_t it = 0, it_end = 0;
+    TileHead head;  // of item `it` while it < it_end
+    head.slot = head.p0 = head.np = 0;
+    head.t = head.my_off = 0;
     bool draining = false;
     // cells of the last clip pass left to the exact queue: this lane's entry and the cells' mask
     unsigned pend = 0u;
@@ -607,16 +638,12 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
                 draining = true;
                 continue;
             }
+            head = load_head(srcs, entries, it, lane);
         }
-        int slot = 0;
-        int64_t t = it;
-        if (entries) {
-            const int2 en = entries[it];
-            slot = __builtin_amdgcn_readfirstlane(en.x);
-            t = int64_t(__builtin_amdgcn_readfirstlane(en.y));
-        }
+        const TileHead cur = head;
         ++it;
-        classify_tile(srcs.get(slot), slot, t, L, lane, n_partial);
+        if (it < it_end) head = load_head(srcs, entries, it, lane);
+        classify_tile(srcs.get(cur.slot), cur, L, lane, n_partial);
     }
     if (lane == 0 && n_partial) atomicAdd(counters + 1, n_partial);
     if (fb) atomicAdd(counters, fb);
