@@ -76,6 +76,7 @@ SIGNATURES = {
     "adeco_selftest_clip_vol6_f32": (ctypes.c_float, [c_vp]),
     "adeco_selftest_classify_f64": (ctypes.c_int, [c_vp]),
     "adeco_selftest_beyond_fraction_f64": (ctypes.c_double, [c_vp]),
+    "adeco_selftest_column_f64": (None, [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp]),
     "adeco_selftest_decomposition": (None, [ctypes.c_int, c_vp]),
     "adeco_selftest_deposit_soup_f64": (ctypes.c_int, [c_vp, ctypes.c_int64, c_vp, ctypes.c_int, ctypes.c_double,
                                                        c_vp]),

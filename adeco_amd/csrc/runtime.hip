@@ -284,6 +284,16 @@ int adeco_selftest_classify_f64(const double* t) {
 
 double adeco_selftest_beyond_fraction_f64(const double* d) { return beyond_fraction(d[0], d[1], d[2], d[3]); }
 
+void adeco_selftest_column_f64(const double* t, int ls, int use_signed, double* vol6_out, double* sab_out) {
+    if (!t || !vol6_out || !sab_out || ls < 1 || ls > 8) return;
+    V3<double> v[4];
+    for (int q = 0; q < 4; ++q) v[q] = mk(t[3 * q], t[3 * q + 1], t[3 * q + 2]);
+    for (int c = 0; c < ls; ++c) vol6_out[c] = sab_out[c] = 0.0;
+    ColumnLeaf<double, 1> leaf{vol6_out, sab_out, ls};
+    if (use_signed) clip_pieces<double, 7>(v, true, leaf);
+    else clip_pieces<double, 6>(v, false, leaf);
+}
+
 adeco_status adeco_selftest_deposit_soup_f64(const float* tets, int64_t n_tets, const float* bbox6, int dims,
                                              double tet_mass, double* grid) {
     if (!tets || !bbox6 || !grid || dims < 1 || !valid_bbox(bbox6)) return fail(ADECO_EINVAL, "selftest: invalid argument");

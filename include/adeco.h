@@ -233,6 +233,11 @@ float adeco_selftest_clip_vol6_f32(const float* tet12);
 int adeco_selftest_classify_f64(const double* tet12);
 double adeco_selftest_beyond_fraction_f64(const double* d4);
 void adeco_selftest_decomposition(int decomposition, int32_t* corners24);
+/* The batched kernel's column-segment clip on the host: tet12 in the segment frame (x, y across
+ * the column, cells [c, c+1] along z for c < ls, 1 <= ls <= 8); vol6_out[c] = 6 x the volume in
+ * cell c summed by the signed (use_signed != 0) or the exact tree, sab_out[c] = the magnitudes it
+ * was summed from. */
+void adeco_selftest_column_f64(const double* tet12, int ls, int use_signed, double* vol6_out, double* sab_out);
 /* The kernels' per-tet FP64 deposit routine run on the host over a tet soup (float [n][4][3]),
  * accumulating into grid (double, dims^3). */
 adeco_status adeco_selftest_deposit_soup_f64(const float* tets, int64_t n_tets, const float* bbox6, int dims,

@@ -118,6 +118,42 @@ def test_kernel_clip_matches_oracle_clip():
         lib.adeco_selftest_clip_vol6_f64(A.ptr(np.ascontiguousarray(t[[1, 0, 2, 3]]))), rel=1e-14)
 
 
+def test_column_segment_clip_matches_oracle_per_cell():
+    """The batched kernel's column-segment clip (one x/y clip for up to 8 cells along z, planes
+    between cells shared) against the oracle's clip of the tet shifted onto each cell.  The exact
+    tree must match every cell; the signed tree every cell it does not flag as possible
+    cancellation noise (those the kernel recomputes with the exact tree)."""
+    lib = A.load()
+    o = O.load()
+    rng = np.random.default_rng(21)
+    flagged = checked = 0
+    for it in range(1500):
+        ls = int(rng.integers(1, 9))
+        scale = 10 ** rng.uniform(-2, 1.0)
+        centre = np.array([rng.uniform(-0.5, 1.5), rng.uniform(-0.5, 1.5), rng.uniform(-1.0, ls + 1.0)])
+        t = centre + scale * rng.standard_normal((4, 3)) * np.array([1.0, 1.0, 2.0])
+        if _vol6(t) < 0:
+            t = t[[1, 0, 2, 3]]
+        t = np.ascontiguousarray(t)
+        tol = 1e-12 * max(1.0, abs(_vol6(t)))
+        want = np.array([6 * o.oracle_clip_unit_cube_ld(O.P(np.ascontiguousarray(t - [0.0, 0.0, c])))
+                         for c in range(ls)])
+        for signed in (0, 1):
+            got = np.zeros(ls)
+            sab = np.zeros(ls)
+            lib.adeco_selftest_column_f64(A.ptr(t), ls, signed, A.ptr(got), A.ptr(sab))
+            for c in range(ls):
+                if signed and not got[c] >= 1e-6 * sab[c]:
+                    flagged += 1
+                    continue
+                checked += 1
+                assert abs(got[c] - want[c]) <= tol, (it, ls, c, signed, got[c], want[c])
+            if not signed:
+                # the exact tree's terms are all positive: the magnitudes are the values
+                assert np.allclose(sab, got, rtol=1e-12, atol=1e-300)
+    assert checked > 10 * flagged
+
+
 def test_beyond_fraction_closed_forms():
     """Fraction of a tet beyond a plane from vertex distances, against exact clipping of a large
     box (the z-slab closed form used by the kernels' clip)."""
