@@ -109,13 +109,10 @@ __device__ __forceinline__ int upper_bound_i64(const int64_t* a, int n, int64_t 
 // (max_seg > 1 and unit cells) a candidate is a segment of up to max_seg cells along the axis of
 // the largest extent (ties -> z, then y); otherwise a single cell.
 // (pair_grid: the grid of every pair, or null to search grid_off)
-__global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorners ct, const int32_t* __restrict__ pairs,
-                                                    int64_t n_pairs, const int64_t* __restrict__ grid_off, int n_grids,
-                                                    const int32_t* __restrict__ pair_grid,
-                                                    const GridDesc* __restrict__ grids, double tet_mass, int max_seg,
-                                                    unsigned* __restrict__ cand, PairData* __restrict__ pd) {
-    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= n_pairs) return;
+__device__ __forceinline__ void pair_setup_one(LatticeView lat, PackedCorners ct, const int32_t* __restrict__ pairs,
+                                               int64_t p, const int64_t* __restrict__ grid_off, int n_grids,
+                                               const int32_t* __restrict__ pair_grid, const GridDesc* __restrict__ grids,
+                                               double tet_mass, int max_seg, unsigned* __restrict__ cand, PairData& o) {
     const int gi = pair_grid ? pair_grid[p] : upper_bound_i64(grid_off, n_grids + 1, p) - 1;
     V3<double> v[4];
     lattice_tet_vertices(lat, ct, pairs[p], v);
@@ -160,7 +157,6 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
         v[0] = v[3];
         v[3] = t;
     }
-    PairData& o = pd[p];
     for (int w = 0; w < 4; ++w) {
         o.S[3 * w] = g.scale[0] * (v[w].x - g.gmin[0]);
         o.S[3 * w + 1] = g.scale[1] * (v[w].y - g.gmin[1]);
@@ -171,6 +167,29 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
     o.grid = gi;
     o.pad_[0] = o.pad_[1] = o.pad_[2] = 0;
     cand[p] = live ? n : 1u;
+}
+
+__global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorners ct, const int32_t* __restrict__ pairs,
+                                                    int64_t n_pairs, const int64_t* __restrict__ grid_off, int n_grids,
+                                                    const int32_t* __restrict__ pair_grid,
+                                                    const GridDesc* __restrict__ grids, double tet_mass, int max_seg,
+                                                    unsigned* __restrict__ cand, PairData* __restrict__ pd) {
+    // the 128-byte lines are assembled in LDS (rows of 17 words: the odd stride spreads a wave's
+    // row writes over the banks) and stored as whole contiguous rows of the block
+    constexpr int kWords = int(sizeof(PairData) / 8);
+    __shared__ unsigned long long rows[256 * (kWords + 1)];
+    const int64_t base = int64_t(blockIdx.x) * blockDim.x;
+    const int64_t p = base + threadIdx.x;
+    if (p < n_pairs) {
+        PairData o;
+        pair_setup_one(lat, ct, pairs, p, grid_off, n_grids, pair_grid, grids, tet_mass, max_seg, cand, o);
+        const unsigned long long* w = reinterpret_cast<const unsigned long long*>(&o);
+        for (int k = 0; k < kWords; ++k) rows[threadIdx.x * (kWords + 1) + k] = w[k];
+    }
+    __syncthreads();
+    const int rows_here = int(min(int64_t(256), n_pairs - base));
+    unsigned long long* dst = reinterpret_cast<unsigned long long*>(pd + base);
+    for (int k = threadIdx.x; k < rows_here * kWords; k += 256) dst[k] = rows[(k / kWords) * (kWords + 1) + k % kWords];
 }
 
 // grid index of every pair of grid-major pairs: one workgroup per grid writes its range
