@@ -131,3 +131,44 @@ def test_c1_single_grid_matches_oracle():
             got = tree.leaf_grids()[0]
             assert O.max_rel_err(got, want, tet_mass=TET_MASS) <= REL
             assert abs(got.sum() - 32 ** 3) <= 1e-9 * 32 ** 3
+
+
+def test_leaf_grids_match_reference_leaf_grids():
+    """The 128^3 adaptive workload in the reference's own geometry (float leaf boxes, 1/dx scaling,
+    its 6-tet decomposition, tet mass 1): for 12 leaves whose float grids the compiled reference
+    produced (tests/golden/ref_leaves_c2.npz, rebuilt bit for bit by the float-faithful oracle),
+    the GPU leaf grids - binned per tet, deposited plain and batched - match the FP64 oracle's
+    deposit of the leaf's meta-patches to 1e-6 per cell, and the reference's float grids to its
+    own float rounding (the reference differs from the FP64 oracle on these leaves by up to 1.5e-3
+    relative per cell, with a floor of 1e-3 of a tet mass, and 1.2e-6 on leaf totals; bars 2e-3
+    and 2e-6)."""
+    import os
+    from adeco_amd.lattice import meta_patches
+    from tests._golden_leaves import reference_leaf_grids
+    path = os.path.join(os.path.dirname(__file__), "golden", "ref_leaves_c2.npz")
+    g = np.load(path, allow_pickle=False)
+    dims = int(g["dims"])
+    lat, d, ref = reference_leaf_grids(g)
+    f64 = []
+    with O.cell_scaling(O.REFERENCE):
+        for node, bbox in zip(g["node"], g["bbox"]):
+            leaf = int(np.flatnonzero(d["dfs"] == node)[0])
+            ids = d["mc_ids"][d["mc_offsets"][leaf]:d["mc_offsets"][leaf + 1]]
+            f64.append(O.deposit_patches_f64(meta_patches(lat, ids), bbox, dims))
+        with A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 50000, 6) as tree:
+            tree.bin(A.DECOMP_REF_EFFECTIVE)
+            dfs = tree.leaves()["dfs"]
+            boxes = tree.leaf_boxes_f64(exact=False)
+            leaves = [int(np.flatnonzero(dfs == node)[0]) for node in g["node"]]
+            for l, bbox in zip(leaves, g["bbox"]):
+                assert np.array_equal(boxes[l].astype(np.float32), bbox)
+            for name, mode in (("plain", A.MODE_PLAIN), ("batched", A.MODE_BATCHED)):
+                print(f"[reference leaves] {name}", flush=True)
+                tree.deposit(dims, decomposition=A.DECOMP_REF_EFFECTIVE, tet_mass=1.0, mode=mode)
+                grids = tree.leaf_grids()
+                for i, l in enumerate(leaves):
+                    ours = grids[l]
+                    assert O.max_rel_err(ours, f64[i], tet_mass=1.0) <= REL, (name, int(g["node"][i]))
+                    r = ref[i].astype(np.float64)
+                    assert abs(ours.sum() - r.sum()) <= 2e-6 * r.sum(), (name, int(g["node"][i]))
+                    assert np.max(np.abs(ours - r) / np.maximum(r, 1e-3)) <= 2e-3, (name, int(g["node"][i]))

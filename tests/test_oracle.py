@@ -21,6 +21,7 @@ import pytest
 import adeco_amd as A
 from adeco_amd.lattice import make_lattice, meta_patches
 from tests import _oracle as O
+from tests._golden_leaves import reference_leaf_grids
 from tests.rational_clip import tet_box_volume
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -259,3 +260,15 @@ def test_effective_decomposition_signs():
             pts = np.array([[(k & 1), (k >> 1) & 1, (k >> 2) & 1] for k in c[4 * t:4 * t + 4]], float)
             got.append(round(vol6(pts)))
         assert got == want
+
+
+def test_float_faithful_oracle_matches_reference_leaf_grids():
+    """Golden leaf grids of the compiled reference (its octree, leaf gather and deposit_tets on 12
+    leaves of the 128^3 workload, tests/golden/make_golden.py): the float-faithful oracle rebuilds
+    every one bit for bit."""
+    g = _golden("ref_leaves_c2.npz")
+    lat, _, grids = reference_leaf_grids(g)
+    assert hashlib.sha256(np.concatenate([lat.x, lat.y, lat.z]).tobytes()).hexdigest() == str(g["lattice_sha"])
+    for i, grid in enumerate(grids):
+        assert hashlib.sha256(grid.tobytes()).hexdigest() == str(g["grid_sha"][i]), int(g["node"][i])
+        assert float(grid.sum(dtype=np.float64)) == float(g["grid_total"][i])
