@@ -124,13 +124,16 @@ __global__ void k_refine_level(const float* __restrict__ mbox, const unsigned* _
 // ---- per-tet binning into leaves (tree descent with a bounded explicit stack) ----
 constexpr int kMaxStack = 8 * kMaxOctreeDepth + 8;
 
+// (the tets of `ids[0..n_tets)`, the tets the leaf-map pass could not take)
 template <bool kFill>
 __global__ void __launch_bounds__(256) k_bin_tets(LatticeView lat, PackedCorners ct, int64_t n_tets,
+                                                  const int32_t* __restrict__ ids,
                                                   const float* __restrict__ node_box, const int32_t* __restrict__ first_child,
                                                   const int32_t* __restrict__ leaf_index, int* __restrict__ leaf_count,
                                                   const int64_t* __restrict__ leaf_offset, int32_t* __restrict__ pairs) {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= n_tets) return;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_tets) return;
+    const int64_t t = ids[i];
     V3<double> v[4];
     lattice_tet_vertices(lat, ct, t, v);
     // the tet AABB in float is exact (vertices are float)
@@ -160,6 +163,119 @@ __global__ void __launch_bounds__(256) k_bin_tets(LatticeView lat, PackedCorners
         }
         for (int c = 0; c < 8 && sp < kMaxStack; ++c)
             if (fstrict(tb, node_box + 6 * (fc + c))) stack[sp++] = fc + c;
+    }
+}
+
+__global__ void k_iota(int32_t* __restrict__ out, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = int32_t(i);
+}
+
+// ---- per-tet binning through a leaf map ----
+// The leaves tile the root; on the grid of the finest leaf level (F^3 cells of edge h = root
+// extent / F) every cell belongs to one leaf.  A tet's candidate leaves are those of the fine cells
+// that its float AABB covers, widened by 1e-4 of a cell each way (a superset of the leaves whose
+// float or exact box it strictly overlaps: those boxes differ from the fine-cell faces by at most
+// an ulp of the coordinates); each candidate is then kept only if the AABB strictly overlaps the
+// leaf's box widened by two float ulps per face, the descent kernel's test.  At most kMapCells
+// fine cells per tet; larger tets go to an overflow list for the descent kernel.  Counter atomics
+// are aggregated per wave: one per distinct leaf among the wave's lanes.
+constexpr int kMapCells = 8;
+
+struct LeafMap {
+    const int32_t* map;       // F^3 leaf indices, i fastest
+    const float* leaf_box;    // per leaf (DFS order), widened as the node boxes
+    int F;
+    double ll[3];
+    double inv_h;
+};
+
+__device__ __forceinline__ unsigned wave_rank(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u));
+}
+
+template <bool kFill>
+__global__ void __launch_bounds__(256) k_bin_map(LatticeView lat, PackedCorners ct, int64_t n_tets, LeafMap lm,
+                                                 const float* __restrict__ root_box, int* __restrict__ leaf_count,
+                                                 const int64_t* __restrict__ leaf_offset, int32_t* __restrict__ pairs,
+                                                 int32_t* __restrict__ overflow, int* __restrict__ n_overflow) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int leaf[kMapCells];
+#pragma unroll
+    for (int c = 0; c < kMapCells; ++c) leaf[c] = -1;
+    bool over = false;
+    if (t < n_tets) {
+        V3<double> v[4];
+        lattice_tet_vertices(lat, ct, t, v);
+        float tb[6];
+        tb[0] = tb[1] = float(v[0].x);
+        tb[2] = tb[3] = float(v[0].y);
+        tb[4] = tb[5] = float(v[0].z);
+        for (int q = 1; q < 4; ++q) {
+            const float p[3] = {float(v[q].x), float(v[q].y), float(v[q].z)};
+            for (int a = 0; a < 3; ++a) {
+                tb[2 * a] = p[a] < tb[2 * a] ? p[a] : tb[2 * a];
+                tb[2 * a + 1] = p[a] > tb[2 * a + 1] ? p[a] : tb[2 * a + 1];
+            }
+        }
+        if (fstrict(tb, root_box)) {
+            int lo[3], n[3];
+            for (int a = 0; a < 3; ++a) {
+                const double u0 = (double(tb[2 * a]) - lm.ll[a]) * lm.inv_h - 1e-4;
+                const double u1 = (double(tb[2 * a + 1]) - lm.ll[a]) * lm.inv_h + 1e-4;
+                const int i0 = u0 <= 0.0 ? 0 : (u0 >= double(lm.F - 1) ? lm.F - 1 : int(u0));
+                const int i1 = u1 <= 0.0 ? 0 : (u1 >= double(lm.F - 1) ? lm.F - 1 : int(u1));
+                lo[a] = i0;
+                n[a] = i1 - i0 + 1;
+            }
+            const int cells = n[0] * n[1] * n[2];
+            if (cells > kMapCells) {
+                over = true;
+            } else {
+                // the fine cells' leaves, each distinct leaf once (fixed slots: registers only)
+#pragma unroll
+                for (int c = 0; c < kMapCells; ++c) {
+                    if (c < cells) {
+                        const int cx = c % n[0], cy = (c / n[0]) % n[1], cz = c / (n[0] * n[1]);
+                        leaf[c] = lm.map[(lo[0] + cx) + int64_t(lm.F) * ((lo[1] + cy) + int64_t(lm.F) * (lo[2] + cz))];
+                    }
+                }
+#pragma unroll
+                for (int c = 1; c < kMapCells; ++c)
+#pragma unroll
+                    for (int d = 0; d < c; ++d)
+                        if (leaf[c] == leaf[d]) leaf[c] = -1;
+#pragma unroll
+                for (int c = 0; c < kMapCells; ++c)
+                    if (leaf[c] >= 0 && !fstrict(tb, lm.leaf_box + 6 * leaf[c])) leaf[c] = -1;
+            }
+        }
+    }
+    // overflow tets: one list append per wave
+    if (!kFill) {
+        const unsigned long long om = __ballot(over);
+        if (om) {
+            const int src = __builtin_ctzll(om);
+            int base = 0;
+            if (lane == src) base = atomicAdd(n_overflow, __popcll(om));
+            base = __shfl(base, src, 64);
+            if (over) overflow[base + int(wave_rank(om))] = int32_t(t);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kMapCells; ++c) {
+        unsigned long long pending = __ballot(leaf[c] >= 0);
+        while (pending) {
+            const int src = __builtin_ctzll(pending);
+            const int L = __shfl(leaf[c], src, 64);
+            const unsigned long long m = __ballot(leaf[c] == L);
+            int base = 0;
+            if (lane == src) base = atomicAdd(leaf_count + L, __popcll(m));
+            base = __shfl(base, src, 64);
+            if (kFill && leaf[c] == L) pairs[leaf_offset[L] + base + int(wave_rank(m))] = int32_t(t);
+            pending &= ~m;
+        }
     }
 }
 
@@ -252,9 +368,71 @@ adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
     const LatticeView lv{L->d_x, L->d_y, L->d_z, {L->np[0], L->np[1], L->np[2]}};
     const PackedCorners ct = pack_corners(corner_table(decomposition));
     const unsigned nb = unsigned((n_tets + 255) / 256);
-    hipLaunchKernelGGL(k_bin_tets<false>, dim3(nb), dim3(256), 0, ds.stream, lv, ct, n_tets, T->d_node_box.as<float>(),
-                       T->d_first.as<int32_t>(), T->d_leaf_index.as<int32_t>(), T->d_leaf_count.as<int>(),
-                       (const int64_t*)nullptr, (int32_t*)nullptr);
+    // the leaf map on the finest leaf level, when that level is at most 7 (F <= 128)
+    int lmax = 0;
+    for (int64_t l = 0; l < n_leaves; ++l) lmax = std::max(lmax, T->nodes[T->leaves_dfs[l]].level);
+    const bool use_map = lmax <= 7;
+    LeafMap lm{};
+    if (use_map) {
+        const int F = 1 << lmax;
+        std::vector<int32_t> map(size_t(F) * F * F, -1);
+        for (int64_t l = 0; l < n_leaves; ++l) {
+            const ONode& n = T->nodes[T->leaves_dfs[l]];
+            const int w = 1 << (lmax - n.level);
+            for (int k = n.ix[2] * w; k < (n.ix[2] + 1) * w; ++k)
+                for (int j = n.ix[1] * w; j < (n.ix[1] + 1) * w; ++j)
+                    for (int i = n.ix[0] * w; i < (n.ix[0] + 1) * w; ++i) map[i + size_t(F) * (j + size_t(F) * k)] = int32_t(l);
+        }
+        for (int32_t m : map)
+            if (m < 0) return fail(ADECO_EINTERNAL, "binning: the leaves do not tile the root");
+        std::vector<float> leaf_box(6 * size_t(n_leaves));
+        for (int64_t l = 0; l < n_leaves; ++l)
+            std::memcpy(&leaf_box[6 * l], &boxes[6 * size_t(T->leaves_dfs[l])], 6 * sizeof(float));
+        if ((st = T->d_leaf_map.reserve(map.size() * 4 + leaf_box.size() * 4)) ||
+            (st = T->d_overflow.reserve(size_t(n_tets) * 4 + 8)))
+            return st;
+        e = hipMemcpyAsync(T->d_leaf_map.ptr, map.data(), map.size() * 4, hipMemcpyHostToDevice, ds.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(T->d_leaf_map.as<char>() + map.size() * 4, leaf_box.data(), leaf_box.size() * 4,
+                               hipMemcpyHostToDevice, ds.stream);
+        if (e == hipSuccess) e = hipMemsetAsync(T->d_overflow.ptr, 0, 8, ds.stream);
+        if (e != hipSuccess) return hip_fail(e, "leaf map upload");
+        lm.map = T->d_leaf_map.as<int32_t>();
+        lm.leaf_box = reinterpret_cast<const float*>(T->d_leaf_map.as<char>() + map.size() * 4);
+        lm.F = F;
+        const double h = double(T->nodes[0].ext) / double(F);
+        for (int a = 0; a < 3; ++a) lm.ll[a] = double(T->nodes[0].ll[a]);
+        lm.inv_h = 1.0 / h;
+    }
+    int* d_nover = use_map ? T->d_overflow.as<int>() : nullptr;
+    int32_t* d_over = use_map ? reinterpret_cast<int32_t*>(T->d_overflow.as<char>() + 8) : nullptr;
+    int n_over = 0;
+    if (use_map) {
+        hipLaunchKernelGGL(k_bin_map<false>, dim3(nb), dim3(256), 0, ds.stream, lv, ct, n_tets, lm, T->d_node_box.as<float>(),
+                           T->d_leaf_count.as<int>(), (const int64_t*)nullptr, (int32_t*)nullptr, d_over, d_nover);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&n_over, d_nover, 4, hipMemcpyDeviceToHost, ds.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
+        if (e != hipSuccess) return hip_fail(e, "binning count pass");
+    } else {
+        if ((st = T->d_overflow.reserve(size_t(n_tets) * 4 + 8))) return st;
+        d_over = reinterpret_cast<int32_t*>(T->d_overflow.as<char>() + 8);
+        hipLaunchKernelGGL(k_iota, dim3(nb), dim3(256), 0, ds.stream, d_over, n_tets);
+        n_over = int(n_tets);
+    }
+    auto descent = [&](bool fill) {
+        if (n_over == 0) return;
+        const unsigned ob = unsigned((int64_t(n_over) + 255) / 256);
+        if (fill)
+            hipLaunchKernelGGL(k_bin_tets<true>, dim3(ob), dim3(256), 0, ds.stream, lv, ct, int64_t(n_over), (const int32_t*)d_over,
+                               T->d_node_box.as<float>(), T->d_first.as<int32_t>(), T->d_leaf_index.as<int32_t>(),
+                               T->d_leaf_count.as<int>(), T->d_leaf_offset.as<int64_t>(), T->d_pairs.as<int32_t>());
+        else
+            hipLaunchKernelGGL(k_bin_tets<false>, dim3(ob), dim3(256), 0, ds.stream, lv, ct, int64_t(n_over), (const int32_t*)d_over,
+                               T->d_node_box.as<float>(), T->d_first.as<int32_t>(), T->d_leaf_index.as<int32_t>(),
+                               T->d_leaf_count.as<int>(), (const int64_t*)nullptr, (int32_t*)nullptr);
+    };
+    descent(false);
     e = hipGetLastError();
     std::vector<int> cnt(n_leaves);
     if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), T->d_leaf_count.ptr, n_leaves * 4, hipMemcpyDeviceToHost, ds.stream);
@@ -267,9 +445,11 @@ adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
     e = hipMemcpyAsync(T->d_leaf_offset.ptr, T->leaf_tet_offset.data(), (n_leaves + 1) * 8, hipMemcpyHostToDevice, ds.stream);
     if (e == hipSuccess) e = hipMemsetAsync(T->d_leaf_count.ptr, 0, n_leaves * 4, ds.stream);
     if (e != hipSuccess) return hip_fail(e, "binning offsets");
-    hipLaunchKernelGGL(k_bin_tets<true>, dim3(nb), dim3(256), 0, ds.stream, lv, ct, n_tets, T->d_node_box.as<float>(),
-                       T->d_first.as<int32_t>(), T->d_leaf_index.as<int32_t>(), T->d_leaf_count.as<int>(),
-                       T->d_leaf_offset.as<int64_t>(), T->d_pairs.as<int32_t>());
+    if (use_map)
+        hipLaunchKernelGGL(k_bin_map<true>, dim3(nb), dim3(256), 0, ds.stream, lv, ct, n_tets, lm, T->d_node_box.as<float>(),
+                           T->d_leaf_count.as<int>(), T->d_leaf_offset.as<int64_t>(), T->d_pairs.as<int32_t>(),
+                           (int32_t*)nullptr, (int*)nullptr);
+    descent(true);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
     if (e != hipSuccess) return hip_fail(e, "binning fill pass");
