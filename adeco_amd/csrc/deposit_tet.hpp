@@ -111,13 +111,28 @@ ADECO_HD void lattice_tet_vertices(const LatticeView& lat, const PackedCorners& 
         for (int q = 0; q < 4; ++q) v[q] = mk(double(p[3 * q]), double(p[3 * q + 1]), double(p[3 * q + 2]));
         return;
     }
-    const int64_t c = t / 6;
-    const int lt = int(t - 6 * c);
     const int64_t ncx = lat.np[0] - 1, ncy = lat.np[1] - 1;
-    const int64_t cj = c / ncx;
-    const int64_t i = c - cj * ncx;
-    const int64_t k = cj / ncy;
-    const int64_t j = cj - k * ncy;
+    int64_t c, i, j, k;
+    int lt;
+    if (t < (int64_t(1) << 32)) {
+        // 32-bit index arithmetic (a 64-bit division is several times longer)
+        const unsigned tt = unsigned(t), cc = tt / 6u;
+        lt = int(tt - 6u * cc);
+        const unsigned cj = cc / unsigned(ncx);
+        const unsigned kk = cj / unsigned(ncy);
+        c = cc;
+        i = int64_t(cc - cj * unsigned(ncx));
+        k = kk;
+        j = int64_t(cj - kk * unsigned(ncy));
+    } else {
+        c = t / 6;
+        lt = int(t - 6 * c);
+        const int64_t cj = c / ncx;
+        i = c - cj * ncx;
+        k = cj / ncy;
+        j = cj - k * ncy;
+    }
+    (void)c;
     for (int q = 0; q < 4; ++q) {
         const int code = ct.code(lt, q);
         const int64_t vi = (i + (code & 1)) +

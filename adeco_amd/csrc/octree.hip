@@ -194,18 +194,25 @@ __device__ __forceinline__ unsigned wave_rank(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u));
 }
 
+// tleaf: per tet, written by the count pass and read by the fill pass: its only leaf, kNoLeaf, or
+// kRecompute (several leaves, or the descent's), so that the fill pass recomputes only those.
+constexpr int32_t kNoLeaf = -2, kRecompute = -1;
+
 template <bool kFill>
 __global__ void __launch_bounds__(256) k_bin_map(LatticeView lat, PackedCorners ct, int64_t n_tets, LeafMap lm,
                                                  const float* __restrict__ root_box, int* __restrict__ leaf_count,
                                                  const int64_t* __restrict__ leaf_offset, int32_t* __restrict__ pairs,
-                                                 int32_t* __restrict__ overflow, int* __restrict__ n_overflow) {
+                                                 int32_t* __restrict__ overflow, int* __restrict__ n_overflow,
+                                                 int32_t* __restrict__ tleaf) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     int leaf[kMapCells];
 #pragma unroll
     for (int c = 0; c < kMapCells; ++c) leaf[c] = -1;
     bool over = false;
-    if (t < n_tets) {
+    const int32_t known = (kFill && t < n_tets) ? tleaf[t] : kRecompute;
+    if (kFill && known >= 0) leaf[0] = known;
+    if (t < n_tets && known == kRecompute) {
         V3<double> v[4];
         lattice_tet_vertices(lat, ct, t, v);
         float tb[6];
@@ -251,6 +258,16 @@ __global__ void __launch_bounds__(256) k_bin_map(LatticeView lat, PackedCorners 
                     if (leaf[c] >= 0 && !fstrict(tb, lm.leaf_box + 6 * leaf[c])) leaf[c] = -1;
             }
         }
+    }
+    if (!kFill && t < n_tets) {
+        int n = 0, only = kNoLeaf;
+#pragma unroll
+        for (int c = 0; c < kMapCells; ++c)
+            if (leaf[c] >= 0) {
+                ++n;
+                only = leaf[c];
+            }
+        tleaf[t] = over || n > 1 ? kRecompute : only;
     }
     // overflow tets: one list append per wave
     if (!kFill) {
@@ -404,12 +421,14 @@ adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
         for (int a = 0; a < 3; ++a) lm.ll[a] = double(T->nodes[0].ll[a]);
         lm.inv_h = 1.0 / h;
     }
+    if (use_map && (st = T->d_tleaf.reserve(size_t(n_tets) * 4))) return st;
+    int32_t* d_tleaf = use_map ? T->d_tleaf.as<int32_t>() : nullptr;
     int* d_nover = use_map ? T->d_overflow.as<int>() : nullptr;
     int32_t* d_over = use_map ? reinterpret_cast<int32_t*>(T->d_overflow.as<char>() + 8) : nullptr;
     int n_over = 0;
     if (use_map) {
         hipLaunchKernelGGL(k_bin_map<false>, dim3(nb), dim3(256), 0, ds.stream, lv, ct, n_tets, lm, T->d_node_box.as<float>(),
-                           T->d_leaf_count.as<int>(), (const int64_t*)nullptr, (int32_t*)nullptr, d_over, d_nover);
+                           T->d_leaf_count.as<int>(), (const int64_t*)nullptr, (int32_t*)nullptr, d_over, d_nover, d_tleaf);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(&n_over, d_nover, 4, hipMemcpyDeviceToHost, ds.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
@@ -448,7 +467,7 @@ adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
     if (use_map)
         hipLaunchKernelGGL(k_bin_map<true>, dim3(nb), dim3(256), 0, ds.stream, lv, ct, n_tets, lm, T->d_node_box.as<float>(),
                            T->d_leaf_count.as<int>(), T->d_leaf_offset.as<int64_t>(), T->d_pairs.as<int32_t>(),
-                           (int32_t*)nullptr, (int*)nullptr);
+                           (int32_t*)nullptr, (int*)nullptr, d_tleaf);
     descent(true);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);

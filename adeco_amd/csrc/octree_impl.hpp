@@ -40,7 +40,8 @@ struct adeco_octree_s {
     int binned_decomposition = -1;
     std::vector<int64_t> leaf_tet_offset;
     adeco::DevBuf d_mc_box, d_mc_cells, d_node_box, d_first, d_leaf_index, d_leaf_count, d_leaf_offset, d_pairs;
-    adeco::DevBuf d_leaf_map, d_overflow;  // binning: leaf of every finest-level cell; tets left to the descent
+    adeco::DevBuf d_leaf_map, d_overflow, d_tleaf;  // binning: leaf of every finest-level cell; tets left
+                                                    // to the descent; each tet's single leaf
     // leaf grids of the last deposit (device, FP64, leaf-major in DFS leaf order)
     int grid_dims = 0;
     adeco::DevBuf d_grids, d_leaf_geom, d_work, d_tiles, d_counters;
