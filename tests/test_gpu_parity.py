@@ -148,3 +148,22 @@ def test_classification_regimes():
     assert st_c[5] > 0 and st_f[6] > 0
     assert O.max_rel_err(coarse, O.deposit_lattice(lat, box, 2, decomp=2, precision=3)) <= REL
     assert O.max_rel_err(fine, O.deposit_lattice(lat, box, 96, decomp=2, precision=3)) <= REL
+
+
+@pytest.mark.parametrize("dims", [24, 96])
+def test_batched_column_segments_on_fine_grids(dims):
+    """The batched FP64 form clips runs of up to 8 cells along each tet's longest axis with one
+    x/y clip.  Tets spanning 4 (dims 24) to 20 (dims 96) cells per axis need several segments per
+    column, full-inside segments and cells in both classification extremes; per cell against the
+    extended-precision oracle, and the same grid as the plain form."""
+    lat = make_lattice(6, 0.5, seed=5)
+    box = padded_box(lat.bbox())
+    tets = O.lattice_tets(lat, A.DECOMP_KUHN)
+    print(f"[columns] dims={dims}", flush=True)
+    got = A.deposit_soup(tets, box, dims, 1.0 / 6, mode=A.MODE_BATCHED, precision=A.PREC_FP64)
+    plain = A.deposit_soup(tets, box, dims, 1.0 / 6, mode=A.MODE_PLAIN, precision=A.PREC_FP64)
+    with O.cell_scaling(O.EXACT):
+        want = O.deposit_soup_ld(tets, box, dims, 1.0 / 6)
+    assert O.max_rel_err(got, want, tet_mass=1.0 / 6) <= REL
+    assert O.max_rel_err(plain, want, tet_mass=1.0 / 6) <= REL
+    assert abs(got.sum() - want.sum()) <= 1e-9 * want.sum()
