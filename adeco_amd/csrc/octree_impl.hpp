@@ -45,7 +45,7 @@ struct adeco_octree_s {
     // leaf grids of the last deposit (device, FP64, leaf-major in DFS leaf order)
     int grid_dims = 0;
     adeco::DevBuf d_grids, d_leaf_geom, d_work, d_tiles, d_counters;
-    int64_t last_candidates = 0;            // (tet, cell) candidates of the last batched deposit
+    int64_t last_candidates = 0;            // (tet, candidate) pairs of the last batched deposit
     unsigned long long* d_fallback = nullptr;  // FP64 recomputes of the last mixed-precision deposit
 
     // (ll, ll + ext) per axis in float, as AdOctreeGeneration::getBBox (AdOctreeGeneration.h:67-73)

@@ -33,7 +33,7 @@ struct DevBuf {
 };
 
 // Timings of the calling thread's last deposit: whole deposit (events on the launch stream),
-// the dominant kernel alone, (tet, cell) candidates and FP64 recomputes of a mixed-precision run.
+// the dominant kernel alone, (tet, candidate) pairs and exact recomputes.
 struct LastTimings {
     double total_ms = 0, main_ms = 0;
     double candidates = 0, fallbacks = 0, partials = 0;

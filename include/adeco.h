@@ -151,7 +151,7 @@ adeco_status adeco_octree_leaf_boxes(adeco_octree tree, int exact, double* out);
 /* Binned lattice tet ids of one leaf (DFS index) after adeco_octree_bin / adeco_deposit_leaves;
  * returns their number (-1 on error); out (nullable) receives them. */
 int64_t adeco_octree_leaf_tets(adeco_octree tree, int64_t leaf, int32_t* out);
-/* Launch shape of the batched kernel: tiles of 64 (tet, cell) candidates taken per work-queue pop
+/* Launch shape of the batched kernel: tiles of 64 (tet, candidate) pairs taken per work-queue pop
  * (1..1024) and resident workgroups of 4 wavefronts per CU (1..16).  Library-wide; set it before
  * concurrent use.  No reference counterpart: the reference's GPU deposit uses a fixed 512 x 64
  * launch (AdDeposit2Grid.h:72-80). */
@@ -189,14 +189,16 @@ typedef struct {
     double tet_mass;      /* mass of one tet (> 0) */
     int max_inflight;     /* requests resident on the device at once, 1..64 */
     int pad_;
-    int64_t step_tiles;   /* tiles of 64 (tet, cell) candidates per step launch (0: 131072) */
+    int64_t step_tiles;   /* tiles of 64 (tet, candidate) pairs per step launch (0: 131072) */
     uint64_t memory_budget; /* device bytes for request buffers (0: no limit but the device's) */
 } adeco_serve_config;
 typedef struct {
     adeco_status status;
     int dims;
     int64_t n_tets;       /* tets selected for the box */
-    int64_t n_candidates; /* (tet, cell) candidates of the batched deposit */
+    int64_t n_candidates; /* (tet, candidate) pairs of the batched deposit (a candidate is a covered
+                           * cell, or for FP64 a run of up to 8 covered cells along the tet's
+                           * longest axis) */
     double t_submit, t_admit, t_done; /* seconds on the server's steady clock */
 } adeco_request_info;
 typedef struct {
