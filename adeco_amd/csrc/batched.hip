@@ -345,10 +345,6 @@ __device__ __forceinline__ unsigned udiv_small(unsigned n, unsigned d) {
     return q;
 }
 
-__device__ __forceinline__ unsigned lane_mask_lt(unsigned long long m) {
-    return __builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u));
-}
-
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -361,8 +357,6 @@ __device__ __forceinline__ int64_t shfl_i64(int64_t v, int src) {
     return int64_t((uint64_t(unsigned(hi)) << 32) | unsigned(lo));
 }
 
-// Classifies the 64 candidates of tile t of `src` (slot `slot` among the launch's sources):
-// trivial cells are deposited, partial cells are appended to the wave's queues.
 // What a tile's classification needs first: its source slot and index, its first pair, the number
 // of pairs it touches and, per lane, the candidate offset of pair p0 + lane.  Loaded one tile
 // ahead, so that these dependent loads overlap the previous tile's work.
@@ -391,6 +385,9 @@ __device__ __forceinline__ TileHead load_head(const Sources& srcs, const int2* _
     return h;
 }
 
+// Classifies the 64 candidates of tile h.t of `src` (slot h.slot among the launch's sources):
+// segments inside the tet and tets inside a single cell are deposited, partial candidates are
+// appended to the wave's queues.
 __device__ __forceinline__ void classify_tile(const TileSource& src, const TileHead& h, WaveLds& L, int lane,
                                               unsigned long long& n_partial) {
     const int slot = h.slot, p0 = h.p0, np = h.np;
