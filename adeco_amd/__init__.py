@@ -13,6 +13,8 @@ All compute runs in HIP kernels on the GPU; the library fails loudly when it or 
 
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from . import _lib
@@ -24,6 +26,7 @@ ADECO_EOVERFLOW = 2
 ADECO_ENOMEM = 3
 ADECO_EDEVICE = 4
 ADECO_EINTERNAL = 5
+ADECO_EIO = 6
 
 DECOMP_REF_EFFECTIVE = 0
 DECOMP_REF_INTENDED = 1
@@ -120,6 +123,30 @@ class DepositWrapper:
                                           int(num_tets_2_process), ptr(positions), int(positions_size),
                                           int(meta_cells_per_patch), ptr(bbox), int(dims), ptr(data))
         _check(st, "deposit_tets()")
+
+
+class AdOctreeGridWriter:
+    """Mirror of AdaptiveMassDeposit::AdOctreeGridWriter (AdOctreeGridWriter.h:54-144): one raw
+    float32 density file per leaf, ``<output_path>/node_<level>_<ix>_<iy>_<iz>``."""
+
+    def write_meta_data(self, output_directory: str, filename: str, linear_patch_resolution: int, octree) -> None:
+        """The reference's base writer stores no meta data (AdOctreeGridWriter.h:58-64)."""
+        print("INFO: AdOctreeGridWriter::store_meta_data(): called for base class. No meta data will be written.")
+
+    def write_patch(self, output_path: str, node: int, octree: "Octree", grid_dims: int, orig_data) -> float:
+        """Writes node ``node`` of ``octree`` from its float mass grid (dims^3, i fastest) as
+        densities and returns the grid's mass (double sum in index order)."""
+        data = np.ascontiguousarray(orig_data, dtype=np.float32)
+        if data.size != int(grid_dims) ** 3:
+            raise AdRuntimeException(ADECO_EINVAL, "ERROR: write_patch(): grid size does not match grid_dims")
+        nd = octree.nodes()
+        ll = np.ascontiguousarray(nd["ll"][int(node)], np.float32)
+        root = np.ascontiguousarray(nd["ll"][0], np.float32)
+        mass = _lib.ctypes.c_double()
+        _check(load().adeco_write_patch(os.fsencode(output_path), int(nd["level"][int(node)]), ptr(ll),
+                                        float(nd["ext"][int(node)]), ptr(root), int(grid_dims), ptr(data),
+                                        _lib.ctypes.byref(mass)), "write_patch()")
+        return mass.value
 
 
 class ResidentLattice:
@@ -310,6 +337,16 @@ class Octree:
         out = np.zeros((count, self.dims ** 3), np.float64)
         _check(load().adeco_leaf_grids(self.handle, int(first), int(count), ptr(out)), "leaf_grids()")
         return out
+
+    def write_leaves(self, output_path: str):
+        """Writes every leaf of the last deposit as a ``node_<level>_<ix>_<iy>_<iz>`` density file
+        (AdOctreeGridWriter::write_patch, AdOctreeGridWriter.h:67-144).  Returns (per-leaf masses in
+        DFS order, total mass summed in the leaf processing order)."""
+        masses = np.zeros(self.num_leaves, np.float64)
+        total = _lib.ctypes.c_double()
+        _check(load().adeco_write_leaves(self.handle, os.fsencode(output_path), ptr(masses), _lib.ctypes.byref(total)),
+               "write_leaves()")
+        return masses, total.value
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:

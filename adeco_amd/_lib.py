@@ -55,6 +55,9 @@ SIGNATURES = {
     "adeco_leaf_grids": (ctypes.c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_vp]),
     "adeco_octree_leaf_tets": (ctypes.c_int64, [c_vp, ctypes.c_int64, c_vp]),
     "adeco_octree_leaf_boxes": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp]),
+    "adeco_write_patch": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, c_vp, ctypes.c_float, c_vp, ctypes.c_int,
+                                         c_vp, c_vp]),
+    "adeco_write_leaves": (ctypes.c_int, [c_vp, ctypes.c_char_p, c_vp, c_vp]),
     "adeco_last_timings": (None, [c_vp]),
     "adeco_set_launch_shape": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "adeco_get_launch_shape": (ctypes.c_int, [c_vp, c_vp]),

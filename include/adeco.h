@@ -16,6 +16,8 @@
  *                            AdTypeDefinitions.h:715-1022, AdOctreeGeneration.h:76-258, AdOctree.h:63-392
  *   adeco_deposit_leaves     deposit_leaf_nodes (one rank)           AdDynamicLoadBalancing.h:983-1323
  *                            + gather_local_tets                     AdDynamicLoadBalancing.h:894-950
+ *   adeco_write_patch        AdOctreeGridWriter::write_patch         AdOctreeGridWriter.h:67-144
+ *   adeco_write_leaves       the write step of deposit_leaf_nodes    AdDynamicLoadBalancing.h:998,1133,1303-1312
  *   adeco_serve_*            (no counterpart: region-of-interest density service on the resident lattice)
  */
 #ifndef ADECO_H
@@ -35,6 +37,7 @@ typedef int adeco_status;
 #define ADECO_ENOMEM 3    /* device or host allocation failed */
 #define ADECO_EDEVICE 4   /* no usable GPU, or a HIP runtime error */
 #define ADECO_EINTERNAL 5 /* internal consistency check failed */
+#define ADECO_EIO 6       /* a file could not be written (the reference throws AdRuntimeException) */
 
 /* Tet decompositions of a Lagrangian lattice cube (6 tets each). */
 #define ADECO_DECOMP_REF_EFFECTIVE 0 /* what the reference deposit applies (AdDepositTets2Grid.h:947-972 x :213-218) */
@@ -151,6 +154,19 @@ adeco_status adeco_octree_leaf_boxes(adeco_octree tree, int exact, double* out);
 /* Binned lattice tet ids of one leaf (DFS index) after adeco_octree_bin / adeco_deposit_leaves;
  * returns their number (-1 on error); out (nullable) receives them. */
 int64_t adeco_octree_leaf_tets(adeco_octree tree, int64_t leaf, int32_t* out);
+/* ---- leaf files: AdOctreeGridWriter::write_patch (AdOctreeGridWriter.h:67-144) ----
+ * Writes <output_path>/node_<level>_<ix>_<iy>_<iz>: grid_dims^3 float32 densities (mass / cell
+ * volume, i fastest, host byte order), (ix,iy,iz) = rint((ll - root_ll) / (ext / grid_dims)).  The
+ * node box is (ll, ll + ext) in float; cell size and volume are double; density[i] =
+ * float(mass[i] * (1 / cell_vol)); *mass_out (nullable) = double sum of mass_grid in index order.
+ * Host-only.  ADECO_EIO when the file cannot be written. */
+adeco_status adeco_write_patch(const char* output_path, int level, const float* node_ll3, float node_ext,
+                               const float* root_ll3, int grid_dims, const float* mass_grid, double* mass_out);
+/* Writes every leaf of the last adeco_deposit_leaves with adeco_write_patch (the FP64 leaf grids
+ * are rounded to float on the device first).  leaf_mass (nullable, num_leaves, DFS order) receives
+ * each leaf's mass; total_mass (nullable) their sum in the leaf processing order, as the
+ * reference's total (AdDynamicLoadBalancing.h:998,1133). */
+adeco_status adeco_write_leaves(adeco_octree tree, const char* output_path, double* leaf_mass, double* total_mass);
 /* Launch shape of the batched kernel: tiles of 64 (tet, candidate) pairs taken per work-queue pop
  * (1..1024) and resident workgroups of 4 wavefronts per CU (1..16).  Library-wide; set it before
  * concurrent use.  No reference counterpart: the reference's GPU deposit uses a fixed 512 x 64

@@ -26,3 +26,27 @@ def reference_leaf_grids(g):
         ids = d["mc_ids"][d["mc_offsets"][leaf]:d["mc_offsets"][leaf + 1]]
         out.append(O.deposit_patches_f32(meta_patches(lat, ids), bbox, dims))
     return lat, d, out
+
+
+def patch_file_name(nodes, node, dims):
+    """write_patch's file name (AdOctreeGridWriter.h:103-124): offsets in cells of the node's grid,
+    with the cell size ext / dims taken in float and the offset rounded to nearest in double."""
+    delta = float(np.float32(nodes["ext"][node]) / np.float32(dims))
+    io = [int(np.rint((float(nodes["ll"][node][a]) - float(nodes["ll"][0][a])) / delta)) for a in range(3)]
+    return f"node_{int(nodes['level'][node])}_{io[0]}_{io[1]}_{io[2]}"
+
+
+def patch_density(nodes, node, dims, mass, exact=False):
+    """write_patch's densities and mass (AdOctreeGridWriter.h:75-98) of a float mass grid: the box
+    (ll, ll + ext) in float, cell sizes and the inverse cell volume in double, float(mass * inv)
+    per cell, and the double mass sum in index order.  exact=True keeps FP64 densities of an FP64
+    grid instead (the checker's side)."""
+    ll = nodes["ll"][node].astype(np.float32)
+    ext = np.float32(nodes["ext"][node])
+    cell = [(float(np.float32(ll[a] + ext)) - float(ll[a])) / dims for a in range(3)]
+    inv = 1.0 / (cell[0] * cell[1] * cell[2])
+    if exact:
+        return np.asarray(mass, np.float64) * inv, float(np.sum(mass))
+    m = np.asarray(mass, np.float32)
+    total = float(np.cumsum(m.astype(np.float64))[-1])  # sequential, in index order
+    return (m.astype(np.float64) * inv).astype(np.float32), total

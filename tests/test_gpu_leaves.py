@@ -141,10 +141,12 @@ def test_leaf_grids_match_reference_leaf_grids():
     deposit of the leaf's meta-patches to 1e-6 per cell, and the reference's float grids to its
     own float rounding (the reference differs from the FP64 oracle on these leaves by up to 1.5e-3
     relative per cell, with a floor of 1e-3 of a tet mass, and 1.2e-6 on leaf totals; bars 2e-3
-    and 2e-6)."""
+    and 2e-6).  The leaf files written from the batched grids are write_patch's files of the
+    float-rounded grids, bit for bit, and match the FP64 oracle's densities to 1e-6 per cell."""
     import os
+    import tempfile
     from adeco_amd.lattice import meta_patches
-    from tests._golden_leaves import reference_leaf_grids
+    from tests._golden_leaves import patch_density, patch_file_name, reference_leaf_grids
     path = os.path.join(os.path.dirname(__file__), "golden", "ref_leaves_c2.npz")
     g = np.load(path, allow_pickle=False)
     dims = int(g["dims"])
@@ -172,3 +174,27 @@ def test_leaf_grids_match_reference_leaf_grids():
                     r = ref[i].astype(np.float64)
                     assert abs(ours.sum() - r.sum()) <= 2e-6 * r.sum(), (name, int(g["node"][i]))
                     assert np.max(np.abs(ours - r) / np.maximum(r, 1e-3)) <= 2e-3, (name, int(g["node"][i]))
+            # leaf files of the batched deposit (AdOctreeGridWriter::write_patch): every leaf's file
+            # is the float rounding of its FP64 grid in the reference's file format, and on the golden
+            # leaves it matches the FP64 oracle's densities to 1e-6 per cell
+            nd = tree.nodes()
+            with tempfile.TemporaryDirectory() as tmp:
+                masses, total = tree.write_leaves(tmp)
+                assert len(os.listdir(tmp)) == tree.num_leaves
+                order = tree.leaves()["order"]
+                want_total = 0.0
+                for node in order:
+                    want_total += masses[int(np.flatnonzero(dfs == node)[0])]
+                assert total == want_total
+                for l in list(range(0, tree.num_leaves, 37)) + leaves:
+                    node = int(dfs[l])
+                    name = patch_file_name(nd, node, dims)
+                    got = np.fromfile(os.path.join(tmp, name), np.float32)
+                    dens, mass = patch_density(nd, node, dims, grids[l].astype(np.float32))
+                    assert np.array_equal(got, dens), name
+                    assert masses[l] == mass, name
+                for i, l in enumerate(leaves):
+                    node = int(dfs[l])
+                    got = np.fromfile(os.path.join(tmp, patch_file_name(nd, node, dims)), np.float32).astype(np.float64)
+                    want, _ = patch_density(nd, node, dims, f64[i], exact=True)
+                    assert O.max_rel_err(got, want, tet_mass=float(want.max())) <= REL, node
