@@ -177,6 +177,24 @@ class ResidentLattice:
         _check(load().adeco_lattice_bbox(self.handle, ptr(out)), "lattice_bbox()")
         return out
 
+    def positions(self):
+        """The resident vertex coordinates (x, y, z), float32 SoA."""
+        n = self.dims[0] * self.dims[1] * self.dims[2]
+        x, y, z = (np.empty(n, np.float32) for _ in range(3))
+        _check(load().adeco_lattice_positions(self.handle, ptr(x), ptr(y), ptr(z)), "lattice_positions()")
+        return x, y, z
+
+    def periodic_unwrap(self, bbox, period=None):
+        """apply_boundary_corrections (AdPointDataReaders.h:156-206) on the device with the global
+        bbox ``bbox``.  Without ``period`` the shift is the bbox extent and ``bbox`` stays the root
+        box of later octrees, as in the reference; with ``period`` (3 floats) vertices move by the
+        period and later octrees use the corrected vertices' bbox."""
+        b = np.ascontiguousarray(bbox, np.float32)
+        per = None if period is None else np.ascontiguousarray(period, np.float32)
+        if b.size != 6 or (per is not None and per.size != 3):
+            raise AdRuntimeException(ADECO_EINVAL, "ERROR: periodic_unwrap(): bbox needs 6 and period 3 values")
+        _check(load().adeco_lattice_periodic_unwrap(self.handle, ptr(b), ptr(per)), "lattice_periodic_unwrap()")
+
     def deposit_grid(self, bbox, dims: int, decomposition: int = DECOMP_REF_EFFECTIVE, tet_mass: float = 1.0,
                      tet_ids=None, mode: int = MODE_PLAIN, precision: int = PREC_FP64, stats: bool = False):
         """Deposits all (or the listed) lattice tets onto one dims^3 FP64 grid; returns it (and stats)."""

@@ -35,6 +35,8 @@ SIGNATURES = {
                                             ctypes.c_int, ctypes.POINTER(c_vp)]),
     "adeco_lattice_destroy": (ctypes.c_int, [c_vp]),
     "adeco_lattice_bbox": (ctypes.c_int, [c_vp, c_vp]),
+    "adeco_lattice_periodic_unwrap": (ctypes.c_int, [c_vp, c_vp, c_vp]),
+    "adeco_lattice_positions": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "adeco_lattice_deposit_grid": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_double, c_vp, ctypes.c_int64,
                                                   c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp, c_vp]),
     "adeco_deposit_soup": (ctypes.c_int, [ctypes.c_int, c_vp, ctypes.c_int64, c_vp, ctypes.c_int,
@@ -58,6 +60,12 @@ SIGNATURES = {
     "adeco_write_patch": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, c_vp, ctypes.c_float, c_vp, ctypes.c_int,
                                          c_vp, c_vp]),
     "adeco_write_leaves": (ctypes.c_int, [c_vp, ctypes.c_char_p, c_vp, c_vp]),
+    "adeco_parameters_init": (None, [c_vp]),
+    "adeco_parse_parameter_file": (ctypes.c_int, [ctypes.c_char_p, c_vp]),
+    "adeco_read_sorted_by_id": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, c_vp, c_vp, c_vp, ctypes.c_int64,
+                                               c_i64p, c_vp]),
+    "adeco_workflow_options_init": (None, [c_vp]),
+    "adeco_deposit_workflow": (ctypes.c_int, [c_vp, c_vp, c_vp]),
     "adeco_last_timings": (None, [c_vp]),
     "adeco_set_launch_shape": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "adeco_get_launch_shape": (ctypes.c_int, [c_vp, c_vp]),
@@ -84,6 +92,30 @@ SIGNATURES = {
     "adeco_selftest_deposit_soup_f64": (ctypes.c_int, [c_vp, ctypes.c_int64, c_vp, ctypes.c_int, ctypes.c_double,
                                                        c_vp]),
 }
+
+
+class Parameters(ctypes.Structure):
+    """adeco_parameters"""
+    _fields_ = [("input_file", ctypes.c_char * 4096), ("output_directory", ctypes.c_char * 4096),
+                ("id_type", ctypes.c_char * 256), ("restart_path", ctypes.c_char * 4096),
+                ("use_regions_of_interest", ctypes.c_int), ("max_refinement_level", ctypes.c_int),
+                ("stride", ctypes.c_int), ("linear_patch_resolution", ctypes.c_int), ("num_pthreads", ctypes.c_int),
+                ("num_mpi_ranks_per_node", ctypes.c_int), ("periodic_boundaries", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("max_tets_per_octree_node", ctypes.c_uint64), ("region_of_interest", ctypes.c_float * 6)]
+
+
+class WorkflowOptions(ctypes.Structure):
+    """adeco_workflow_options"""
+    _fields_ = [("device", ctypes.c_int), ("mode", ctypes.c_int), ("precision", ctypes.c_int),
+                ("decomposition", ctypes.c_int), ("tet_mass", ctypes.c_double)]
+
+
+class WorkflowResult(ctypes.Structure):
+    """adeco_workflow_result"""
+    _fields_ = [("num_particles", ctypes.c_int64), ("particle_dims", ctypes.c_int * 3),
+                ("global_bbox", ctypes.c_float * 6), ("num_nodes", ctypes.c_int64), ("num_leaves", ctypes.c_int64),
+                ("total_mass", ctypes.c_double), ("load_s", ctypes.c_double), ("octree_s", ctypes.c_double),
+                ("deposit_s", ctypes.c_double), ("write_s", ctypes.c_double), ("kernel_ms", ctypes.c_double)]
 
 
 class ServeConfig(ctypes.Structure):

@@ -47,6 +47,33 @@ __global__ void __launch_bounds__(256) k_minmax(const float* __restrict__ x, con
     if (threadIdx.x < 6) out[6 * blockIdx.x + threadIdx.x] = sm[threadIdx.x][0];
 }
 
+// apply_boundary_corrections (AdPointDataReaders.h:156-206) per vertex: along each axis, a vertex
+// in the upper half of the lattice (index > dims/2) lying below the box's first quarter mark moves
+// up by the shift; one in the lower half lying above the third quarter mark moves down.  The
+// thresholds come from the host in float exactly as the reference forms them, so the kernel does
+// only compares and one float add or subtract per moved coordinate.
+struct UnwrapParams {
+    float lo[3], hi[3], half[3], shift[3];
+    int np[3];
+};
+
+__global__ void __launch_bounds__(256) k_periodic_unwrap(float* __restrict__ x, float* __restrict__ y,
+                                                         float* __restrict__ z, int64_t n, UnwrapParams p) {
+    for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t plane = int64_t(p.np[0]) * p.np[1];
+        const int idx[3] = {int(v % p.np[0]), int((v / p.np[0]) % p.np[1]), int(v / plane)};
+        float* c[3] = {x, y, z};
+        for (int a = 0; a < 3; ++a) {
+            const float pos = c[a][v];
+            const float fi = float(idx[a]);
+            if (fi > p.half[a] && pos < p.lo[a])
+                c[a][v] = pos + p.shift[a];
+            else if (fi < p.half[a] && pos > p.hi[a])
+                c[a][v] = pos - p.shift[a];
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_iota_i32(int32_t* out, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) out[i] = int32_t(i);
@@ -190,6 +217,51 @@ adeco_status adeco_lattice_bbox(adeco_lattice L, float* bbox6) {
     hipError_t e = hipSetDevice(L->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     return lattice_bbox_device(L, bbox6);
+}
+
+adeco_status adeco_lattice_positions(adeco_lattice L, float* x, float* y, float* z) {
+    if (!L || !x || !y || !z) return fail(ADECO_EINVAL, "lattice_positions(): null argument");
+    std::lock_guard<std::mutex> lk(L->ds->mutex);
+    hipError_t e = hipSetDevice(L->device);
+    const size_t bytes = size_t(L->n_vertices) * sizeof(float);
+    if (e == hipSuccess) e = hipMemcpy(x, L->d_x, bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(y, L->d_y, bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(z, L->d_z, bytes, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? ADECO_OK : hip_fail(e, "lattice positions download");
+}
+
+adeco_status adeco_lattice_periodic_unwrap(adeco_lattice L, const float* bbox6, const float* period3) {
+#pragma clang fp contract(off)
+    if (!L || !bbox6) return fail(ADECO_EINVAL, "lattice_periodic_unwrap(): null argument");
+    for (int a = 0; a < 3; ++a) {
+        if (!(bbox6[2 * a] <= bbox6[2 * a + 1])) return fail(ADECO_EINVAL, "lattice_periodic_unwrap(): invalid bbox");
+        if (period3 && !(period3[a] > 0.f && std::isfinite(period3[a])))
+            return fail(ADECO_EINVAL, "lattice_periodic_unwrap(): period must be positive and finite");
+    }
+    // thresholds as AdPointDataReaders.h:159-162 forms them, in float
+    UnwrapParams p;
+    for (int a = 0; a < 3; ++a) {
+        const float mn = bbox6[2 * a];
+        const float ext = bbox6[2 * a + 1] - mn;
+        p.lo[a] = mn + ext / 4.0f;
+        p.hi[a] = mn + 0.75f * ext;
+        p.half[a] = float(L->np[a]) / 2.0f;
+        p.shift[a] = period3 ? period3[a] : ext;
+        p.np[a] = L->np[a];
+    }
+    std::lock_guard<std::mutex> lk(L->ds->mutex);
+    hipError_t e = hipSetDevice(L->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    DeviceState& ds = *L->ds;
+    const unsigned nb = unsigned(std::min<int64_t>(L->n_vertices / 256 + 1, 4096));
+    hipLaunchKernelGGL(k_periodic_unwrap, dim3(nb), dim3(256), 0, ds.stream, L->d_x, L->d_y, L->d_z, L->n_vertices, p);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "periodic unwrap");
+    L->meta_box.clear();  // vertices moved
+    L->has_domain = !period3;
+    if (!period3) std::memcpy(L->domain, bbox6, sizeof(L->domain));
+    return ADECO_OK;
 }
 
 adeco_status adeco_lattice_deposit_grid(adeco_lattice L, int decomposition, double tet_mass,

@@ -18,6 +18,11 @@ struct adeco_lattice_s {
     float* d_z = nullptr;
     // float AABBs of the meta-cells (7^3 lattice cubes), [M][6], filled by the first ROI selection
     std::vector<float> meta_box;
+    // The global bbox octrees are rooted at, when it is not the vertices' own bbox: the reference's
+    // periodic correction moves vertices but keeps the bbox computed before it
+    // (AdDepositWorkFlow.h:101-112).
+    bool has_domain = false;
+    float domain[6] = {0, 0, 0, 0, 0, 0};
     int meta_dims[3] = {0, 0, 0};
     ~adeco_lattice_s() {
         if (d_x) (void)hipFree(d_x);

@@ -55,8 +55,11 @@ void box_of(const BNode& n, float out[6]) {
 adeco_status build_tree(adeco_octree_s* T, const float* roi6, uint64_t max_tets_param, int max_level) {
     adeco_lattice_s* L = T->L;
     DeviceState& ds = *L->ds;
-    adeco_status st = lattice_bbox_device(L, T->root_bbox);
-    if (st) return st;
+    adeco_status st = ADECO_OK;
+    if (L->has_domain)
+        std::memcpy(T->root_bbox, L->domain, sizeof(T->root_bbox));
+    else if ((st = lattice_bbox_device(L, T->root_bbox)))
+        return st;
     int md[3];
     for (int a = 0; a < 3; ++a) md[a] = (L->np[a] - 1) / 7 + ((L->np[a] - 1) % 7 > 0);  // ceil_div(np-1, 7)
     const int M = md[0] * md[1] * md[2];

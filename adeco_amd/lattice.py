@@ -95,6 +95,29 @@ def make_lattice(n: int, rms_spacings: float = 1.0, seed: int = 1, box: float = 
                    z=pz.astype(np.float32).ravel())
 
 
+def make_snapshot(n: int, rms_spacings: float = 1.0, seed: int = 1, box: float = BOX,
+                  sigma_spacings: float = 2.0) -> Lattice:
+    """A periodic particle snapshot as a SORTED_BY_ID file holds it: n^3 particles, particle id
+    i + n*(j + n*k) at its lattice site plus the displacement of make_lattice, wrapped into
+    [0, box).  Returned as a Lattice of n^3 vertices (n - 1 cubes per axis), ids in order."""
+    if n < 2:
+        raise ValueError("n must be >= 2")
+    full = make_lattice(n, rms_spacings, seed=seed, box=box, sigma_spacings=sigma_spacings)
+    out = []
+    for c in (full.x, full.y, full.z):
+        v = c.reshape(n + 1, n + 1, n + 1)[:n, :n, :n].astype(np.float64)
+        w = np.mod(v, box).astype(np.float32)
+        w[w >= np.float32(box)] = 0.0
+        out.append(np.ascontiguousarray(w.ravel()))
+    return Lattice(n=n - 1, box=box, x=out[0], y=out[1], z=out[2])
+
+
+def write_sorted_by_id(path: str, lat: Lattice) -> None:
+    """Writes the SORTED_BY_ID format: three float32 coordinates per particle, particles in id
+    order (AdPointDataReaders.h:770-911)."""
+    np.stack([lat.x, lat.y, lat.z], axis=1).astype(np.float32).tofile(path)
+
+
 def _apply_infall(px, py, pz, box, count, rng):
     """Spherical infall: x -> x - A (x - c) w(|x - c| / R) with w(r) = (1 - r^2)^2 for r < 1.
 

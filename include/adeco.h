@@ -12,10 +12,14 @@
  *                            (CPU AdDeposit2Grid.cpp:67-125, CUDA AdDeposit2Grid.cu:229-584)
  *   adeco_get_grid_size      DepositWrapper::get_cuda_grid_size      AdDeposit2Grid.h:72-75
  *   adeco_get_block_size     DepositWrapper::get_cuda_block_size     AdDeposit2Grid.h:77-80
+ *   adeco_lattice_periodic_unwrap  apply_boundary_corrections      AdPointDataReaders.h:156-206
  *   adeco_octree_*           MetaMesh + AdOctreeGeneration::generate_tree + Octree
  *                            AdTypeDefinitions.h:715-1022, AdOctreeGeneration.h:76-258, AdOctree.h:63-392
  *   adeco_deposit_leaves     deposit_leaf_nodes (one rank)           AdDynamicLoadBalancing.h:983-1323
  *                            + gather_local_tets                     AdDynamicLoadBalancing.h:894-950
+ *   adeco_parse_parameter_file  parse_parameter_file                 main.cpp:83-183
+ *   adeco_read_sorted_by_id  AdSortedByIdsPointDataReader::load_data AdPointDataReaders.h:770-911
+ *   adeco_deposit_workflow   deposit_workflow (one rank)             AdDepositWorkFlow.h:60-237
  *   adeco_write_patch        AdOctreeGridWriter::write_patch         AdOctreeGridWriter.h:67-144
  *   adeco_write_leaves       the write step of deposit_leaf_nodes    AdDynamicLoadBalancing.h:998,1133,1303-1312
  *   adeco_serve_*            (no counterpart: region-of-interest density service on the resident lattice)
@@ -93,6 +97,18 @@ adeco_status adeco_lattice_create(int device_id, const float* x, const float* y,
 adeco_status adeco_lattice_destroy(adeco_lattice lat);
 /* Float min/max over all vertices (AdPointDataReaders.h:90-151), computed on the device. */
 adeco_status adeco_lattice_bbox(adeco_lattice lat, float* bbox6);
+
+/* Copies the resident vertex coordinates (SoA, npx*npy*npz floats each) back to the host. */
+adeco_status adeco_lattice_positions(adeco_lattice lat, float* x, float* y, float* z);
+/* Periodic-boundary correction on the device, as apply_boundary_corrections
+ * (AdPointDataReaders.h:156-206) with the global bbox bbox6 (normally adeco_lattice_bbox before
+ * the call; vertex ids are the ColumnMajorOrder lattice index): along each axis a vertex with index
+ * > n/2 lying below min + ext/4 moves up by the shift, one with index < n/2 lying above
+ * min + 3/4 ext moves down.  period3 null: the reference's shift, the bbox extent, and bbox6 stays
+ * the global bbox that later octrees are rooted at (the reference does not recompute it,
+ * AdDepositWorkFlow.h:101-112).  period3 given: shift by the period per axis, and later octrees are
+ * rooted at the corrected vertices' bbox.  Call before building octrees or servers on the lattice. */
+adeco_status adeco_lattice_periodic_unwrap(adeco_lattice lat, const float* bbox6, const float* period3);
 
 /* Deposits lattice tets (all 6*(npx-1)(npy-1)(npz-1), or the listed tet ids) with tet_mass onto
  * one dims^3 grid over bbox6; grid_out (host, double, dims^3) is overwritten.  stats (nullable,
@@ -192,6 +208,65 @@ adeco_status adeco_deposit_graph_destroy(adeco_deposit_graph graph);
 /* The calling thread's last leaf deposit: out5 = {total ms, dominant-kernel ms, (tet, cell)
  * candidates, partial cells clipped, FP64 recomputes of the mixed-precision clip}. */
 void adeco_last_timings(double* out5);
+
+/* ---- the single-rank program: parameter file, SORTED_BY_ID reader, deposit_workflow ----
+ * AMD_Parameters (AdTypeDefinitions.h:1218-1247) as a plain struct.  adeco_parameters_init sets
+ * the defaults the reference's constructor and parser set (max_refinement_level 5, stride 1,
+ * linear_patch_resolution 64, one thread and rank, periodic_boundaries on, no tet limit,
+ * region_of_interest (0,1)^3 unused). */
+typedef struct {
+    char input_file[4096];
+    char output_directory[4096];
+    char id_type[256];         /* file_format / id_type: SORTED_BY_ID */
+    char restart_path[4096];   /* parsed, unused (as in the reference) */
+    int use_regions_of_interest;
+    int max_refinement_level;
+    int stride;                /* parsed; the SORTED_BY_ID reader ignores it (as the reference) */
+    int linear_patch_resolution;
+    int num_pthreads;          /* parsed; one GPU does the work of the reference's threads */
+    int num_mpi_ranks_per_node;
+    int periodic_boundaries;
+    int pad_;
+    uint64_t max_tets_per_octree_node;
+    float region_of_interest[6]; /* x0,x1,y0,y1,z0,z1 */
+} adeco_parameters;
+void adeco_parameters_init(adeco_parameters* p);
+/* parse_parameter_file (main.cpp:83-183): "key = value" lines, the key being the text before '='
+ * less its last character, matched as a substring in the reference's order; flags are on when the
+ * value contains "true"; numbers are read as by istream extraction; at most 10000 lines.
+ * ADECO_EINVAL with the reference's message on an unreadable file, a bad number or a
+ * region_of_interest without 6 values; ADECO_EOVERFLOW for a string longer than its field. */
+adeco_status adeco_parse_parameter_file(const char* filename, adeco_parameters* out);
+/* AdSortedByIdsPointDataReader::load_data (AdPointDataReaders.h:770-911) on one rank: raw float32
+ * x,y,z per particle, in id order, n^3 particles of a cubic lattice (n = rint(cbrt(count))).
+ * *num_particles and dims3 receive the count and lattice; when x, y and z are given (capacity >=
+ * count) they receive the coordinates (SoA, ColumnMajorOrder).  stride must be >= 1 and is
+ * otherwise ignored, as in the reference. */
+adeco_status adeco_read_sorted_by_id(const char* filename, int stride, float* x, float* y, float* z, int64_t capacity,
+                                     int64_t* num_particles, int* dims3);
+typedef struct {
+    int device;
+    int mode;           /* ADECO_MODE_* (default batched) */
+    int precision;      /* ADECO_PREC_* (default FP64) */
+    int decomposition;  /* ADECO_DECOMP_* (default REF_EFFECTIVE, the reference's tets) */
+    double tet_mass;    /* default 1 (AdDepositTets2Grid.h:1007-1053) */
+} adeco_workflow_options;
+void adeco_workflow_options_init(adeco_workflow_options* o);
+typedef struct {
+    int64_t num_particles;
+    int particle_dims[3];
+    float global_bbox[6];  /* before the periodic correction: the octree's root box */
+    int64_t num_nodes, num_leaves;
+    double total_mass;     /* sum of the leaf files' masses in the leaf processing order */
+    double load_s, octree_s, deposit_s, write_s; /* wall time of the steps */
+    double kernel_ms;      /* the leaf deposit's kernel time */
+} adeco_workflow_result;
+/* deposit_workflow (AdDepositWorkFlow.h:60-237) on one rank and one GPU: read the particles,
+ * upload them, global bbox, periodic correction when periodic_boundaries is set, octree over the
+ * region of interest (the global bbox unless use_regions_of_interest), leaf deposit at
+ * linear_patch_resolution and one write_patch file per leaf into output_directory. */
+adeco_status adeco_deposit_workflow(const adeco_parameters* p, const adeco_workflow_options* opt,
+                                    adeco_workflow_result* res);
 
 /* ---- region-of-interest density requests over the resident lattice ----
  * The reference has no request server: these entry points apply its single-rank leaf deposit

@@ -277,12 +277,17 @@ struct OracleOctree {
     OTree tree;
 };
 
-void* oracle_octree_build(const float* x, const float* y, const float* z, const int* np,
-                          const float* roi6, uint64_t max_tets_param, int max_level) {
+// The octree of generate_tree (AdOctreeGeneration.h:76-208) rooted at box6 (x0,x1,y0,y1,z0,z1),
+// or at the vertices' float bbox when box6 is null (the workflow's global bbox,
+// AdDepositWorkFlow.h:101; after a periodic correction the reference keeps the bbox from before).
+void* oracle_octree_build_in(const float* x, const float* y, const float* z, const int* np, const float* box6,
+                             const float* roi6, uint64_t max_tets_param, int max_level) {
     auto* h = new OracleOctree();
     const int64_t n = int64_t(np[0]) * np[1] * np[2];
     h->mc = build_meta_cells(x, y, z, np);
-    const FBox box = lattice_bbox(x, y, z, n);
+    FBox box = lattice_bbox(x, y, z, n);
+    if (box6)
+        for (int a = 0; a < 3; ++a) { box.mn[a] = box6[2 * a]; box.mx[a] = box6[2 * a + 1]; }
     const uint64_t num_tets = 6ull * uint64_t(np[0] - 1) * uint64_t(np[1] - 1) * uint64_t(np[2] - 1);
     const uint64_t max_tets = std::min(num_tets, max_tets_param);  // AdDepositWorkFlow.h:169
     FBox roi;
@@ -293,7 +298,39 @@ void* oracle_octree_build(const float* x, const float* y, const float* z, const 
     h->tree = build_octree(h->mc, box, roi6 ? &roi : nullptr, max_tets, max_level);
     return h;
 }
+void* oracle_octree_build(const float* x, const float* y, const float* z, const int* np,
+                          const float* roi6, uint64_t max_tets_param, int max_level) {
+    return oracle_octree_build_in(x, y, z, np, nullptr, roi6, max_tets_param, max_level);
+}
 void oracle_octree_free(void* h) { delete static_cast<OracleOctree*>(h); }
+
+// ------------------------------------------------------------------ periodic correction
+// apply_boundary_corrections (AdPointDataReaders.h:156-206) on an (np[0],np[1],np[2]) lattice whose
+// vertex ids are the ColumnMajorOrder index (AdTypeDefinitions.h:193-220): per axis, ext = max - min,
+// first and third quarter marks min + ext/4 and min + (3/4) ext, half = n/2, all in float; a vertex
+// with index > half below the first mark moves up by the shift, one with index < half above the
+// third mark moves down.  shift = ext (the reference) or period3[a] when period3 is given.
+void oracle_periodic_unwrap(float* x, float* y, float* z, const int* np, const float* bbox6, const float* period3) {
+    float lo[3], hi[3], half[3], shift[3];
+    for (int a = 0; a < 3; ++a) {
+        const float mn = bbox6[2 * a], ext = bbox6[2 * a + 1] - bbox6[2 * a];
+        lo[a] = mn + ext / 4.0f;
+        hi[a] = mn + 0.75f * ext;
+        half[a] = float(np[a]) / 2.0f;
+        shift[a] = period3 ? period3[a] : ext;
+    }
+    float* c[3] = {x, y, z};
+    int64_t v = 0;
+    for (int k = 0; k < np[2]; ++k)
+        for (int j = 0; j < np[1]; ++j)
+            for (int i = 0; i < np[0]; ++i, ++v) {
+                const int idx[3] = {i, j, k};
+                for (int a = 0; a < 3; ++a) {
+                    if (float(idx[a]) > half[a] && c[a][v] < lo[a]) c[a][v] += shift[a];
+                    else if (float(idx[a]) < half[a] && c[a][v] > hi[a]) c[a][v] -= shift[a];
+                }
+            }
+}
 int64_t oracle_octree_num_nodes(void* h) { return int64_t(static_cast<OracleOctree*>(h)->tree.nodes.size()); }
 int64_t oracle_octree_num_leaves(void* h) { return int64_t(static_cast<OracleOctree*>(h)->tree.leaves_dfs.size()); }
 int64_t oracle_octree_num_meta_cells(void* h) { return int64_t(static_cast<OracleOctree*>(h)->mc.bbox.size()); }

@@ -14,6 +14,11 @@
 //   ref_octree_deposit_leaves -> gather_local_tets (AdDynamicLoadBalancing.h:894-950) + deposit_tets per
 //                            leaf, single worker (the 1-thread scheduler result, AdDynamicLoadBalancing.h:384-540)
 //   ref_write_patch_mass  -> AdOctreeGridWriter::write_patch mass sum / density (AdOctreeGridWriter.h:67-144)
+//   ref_parse_parameter_file -> parse_parameter_file (main.cpp:83-183)
+//   ref_cli_main          -> the reference program's main (main.cpp:187-249): parameter file, MPI
+//                            start-up, deposit_workflow (AdDepositWorkFlow.h:60-237), MPI_Finalize
+//   ref_apply_boundary_corrections -> apply_boundary_corrections (AdPointDataReaders.h:156-206)
+//   ref_sorted_by_id_load -> AdSortedByIdsPointDataReader::load_data (AdPointDataReaders.h:770-911)
 
 #include <mpi.h>
 #include <fstream>
@@ -34,6 +39,11 @@
 #include "AdDynamicLoadBalancing.h"
 
 using namespace AdaptiveMassDeposit;
+
+// Defined in the reference's main.cpp, which build_ref.sh compiles with its main renamed to
+// adeco_ref_cli_main.
+int adeco_ref_cli_main(int argc, char** argv);
+void parse_parameter_file(const std::string& filename, AMD_Parameters& parameters);
 
 namespace {
 
@@ -221,6 +231,104 @@ int ref_octree_deposit_leaf(int32_t node_id, int dims, float* grid, double* mass
         return -1;
     }
     return 0;
+}
+
+// Runs the reference program on a parameter file, as `dm_resampling_cpu <param_file>` would.  MPI
+// is finalised at its end and cannot start again, so callers run this in a process of its own.
+int ref_cli_main(const char* param_file) {
+    std::string prog("dm_resampling_cpu"), arg(param_file);
+    std::vector<char> a0(prog.begin(), prog.end()), a1(arg.begin(), arg.end());
+    a0.push_back(0);
+    a1.push_back(0);
+    char* argv[] = {a0.data(), a1.data(), nullptr};
+    return adeco_ref_cli_main(2, argv);
+}
+
+// The reference's parameter-file parser.  strs receives input_file, output_directory, id_type and
+// restart_path, cap bytes each (NUL-terminated, truncated); ints = {use_regions_of_interest,
+// max_refinement_level, stride, max_tets_per_octree_node, linear_patch_resolution, num_pthreads,
+// num_mpi_ranks_per_node, periodic_boundaries}; roi6 = x0,x1,y0,y1,z0,z1.
+int ref_parse_parameter_file(const char* path, char* strs, int cap, int64_t* ints, float* roi6) {
+    try {
+        AMD_Parameters p;
+        parse_parameter_file(path, p);
+        const std::string* s[4] = {&p.input_file, &p.output_directory, &p.id_type, &p.restart_path};
+        for (int i = 0; i < 4; ++i) {
+            const size_t n = std::min(s[i]->size(), size_t(cap - 1));
+            std::memcpy(strs + size_t(i) * cap, s[i]->data(), n);
+            strs[size_t(i) * cap + n] = 0;
+        }
+        ints[0] = p.use_regions_of_interest;
+        ints[1] = p.max_refinement_level;
+        ints[2] = p.stride;
+        ints[3] = int64_t(p.max_tets_per_octree_node);
+        ints[4] = p.linear_patch_resolution;
+        ints[5] = p.num_pthreads;
+        ints[6] = p.num_mpi_ranks_per_node;
+        ints[7] = p.periodic_boundaries;
+        for (int a = 0; a < 3; ++a) {
+            roi6[2 * a] = p.region_of_interest.min[a];
+            roi6[2 * a + 1] = p.region_of_interest.max[a];
+        }
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+    return 0;
+}
+
+// Periodic-boundary correction of an (npx,npy,npz) lattice (ids = ColumnMajorOrder linear index)
+// with the given global bbox; x, y, z are corrected in place.
+int ref_apply_boundary_corrections(float* x, float* y, float* z, int npx, int npy, int npz, const float* b) {
+    try {
+        const size_t n = size_t(npx) * size_t(npy) * size_t(npz);
+        Particles p;
+        for (size_t i = 0; i < n; ++i) p.push_back(x[i], y[i], z[i], ids_t(i));
+        const AABBox box(PosVec(b[0], b[2], b[4]), PosVec(b[1], b[3], b[5]));
+        apply_boundary_corrections(box, AvVec3i(npx, npy, npz), ColumnMajorOrder(), p);
+        const Particles::pos_t_vec& px = *p.get_positions(0);
+        const Particles::pos_t_vec& py = *p.get_positions(1);
+        const Particles::pos_t_vec& pz = *p.get_positions(2);
+        for (size_t i = 0; i < n; ++i) {
+            x[i] = px[i];
+            y[i] = py[i];
+            z[i] = pz[i];
+        }
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+    return 0;
+}
+
+// Loads a SORTED_BY_ID file on one rank.  Returns the particle count (-1 on error); with cap at
+// least that count, x, y, z and ids receive the particles in load order and dims3 the lattice.
+int64_t ref_sorted_by_id_load(const char* path, int64_t cap, float* x, float* y, float* z, int64_t* ids, int* dims3) {
+    try {
+        ensure_mpi();
+        std::shared_ptr<AdPointDataReader> reader = get_reader("SORTED_BY_ID");
+        reader->load_data(path, 0, 1, 1);
+        const Particles& p = *reader->get_particles_per_rank()[0];
+        const int64_t n = int64_t(p.get_num());
+        if (n <= cap) {
+            for (int64_t i = 0; i < n; ++i) {
+                pos_t pos[3];
+                p.get_position(size_t(i), pos);
+                x[i] = pos[0];
+                y[i] = pos[1];
+                z[i] = pos[2];
+                ids[i] = int64_t(p.get_id(size_t(i)));
+            }
+            const AvVec3i d = reader->get_global_particle_dims();
+            dims3[0] = d[0];
+            dims3[1] = d[1];
+            dims3[2] = d[2];
+        }
+        return n;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
 }
 
 }  // extern "C"

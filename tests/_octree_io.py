@@ -7,11 +7,15 @@ import numpy as np
 from tests import _oracle as O
 
 
-def oracle_octree(lat, max_tets, max_level, roi=None):
+def oracle_octree(lat, max_tets, max_level, roi=None, root_box=None):
+    """The oracle's octree; root_box (6 floats) roots it at a given global bbox instead of the
+    vertices' own (the periodic workflow keeps the bbox from before the correction)."""
     o = O.load()
     npd = np.array(lat.dims, np.int32)
     roi = None if roi is None else np.ascontiguousarray(roi, np.float32)
-    h = o.oracle_octree_build(O.P(lat.x), O.P(lat.y), O.P(lat.z), O.P(npd), O.P(roi), max_tets, max_level)
+    box = None if root_box is None else np.ascontiguousarray(root_box, np.float32)
+    h = o.oracle_octree_build_in(O.P(lat.x), O.P(lat.y), O.P(lat.z), O.P(npd), O.P(box), O.P(roi), max_tets,
+                                 max_level)
     try:
         n = o.oracle_octree_num_nodes(h)
         nl = o.oracle_octree_num_leaves(h)

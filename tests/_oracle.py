@@ -34,6 +34,8 @@ _ORACLE_SIGS = {
     "oracle_vol6_f64": (dbl, [vp]),
     "oracle_decomposition": (None, [cint, vp]),
     "oracle_octree_build": (vp, [vp, vp, vp, vp, vp, u64, cint]),
+    "oracle_octree_build_in": (vp, [vp, vp, vp, vp, vp, vp, u64, cint]),
+    "oracle_periodic_unwrap": (None, [vp, vp, vp, vp, vp, vp]),
     "oracle_octree_free": (None, [vp]),
     "oracle_octree_num_nodes": (i64, [vp]),
     "oracle_octree_num_leaves": (i64, [vp]),
@@ -87,6 +89,10 @@ _REF_SIGS = {
     "ref_meta_cell_bboxes": (None, [vp]),
     "ref_estimate": (u64, [vp, i64]),
     "ref_octree_deposit_leaf": (cint, [ctypes.c_int32, cint, vp, vp, vp, ctypes.c_char_p]),
+    "ref_cli_main": (cint, [ctypes.c_char_p]),
+    "ref_parse_parameter_file": (cint, [ctypes.c_char_p, vp, cint, vp, vp]),
+    "ref_apply_boundary_corrections": (cint, [vp, vp, vp, cint, cint, cint, vp]),
+    "ref_sorted_by_id_load": (i64, [ctypes.c_char_p, i64, vp, vp, vp, vp, vp]),
 }
 
 _cache = {}
