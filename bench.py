@@ -353,6 +353,90 @@ def heavy_tail_study(dims_list=(16, 32), n=256, steps=3):
     return out
 
 
+def program_study(n=N, runs=3, with_reference=True):
+    """The reference program's single-rank workflow end to end on one GPU (adeco_deposit_workflow,
+    the body of adeco_resample): a periodic n^3-particle SORTED_BY_ID snapshot (k^-2 field at rms
+    3 spacings, wrapped into the box) read from a file, uploaded, periodic correction, octree
+    (depth <= 6, split above 50,000 estimated tets), batched FP64 leaf deposit of the reference's
+    tets (mass 1) at 16^3 cells per leaf, and one write_patch file per leaf.  One untimed run,
+    then `runs` timed runs; step times are the program's own.  with_reference: also time the
+    reference program on the same input (reference_program_time)."""
+    import tempfile
+    from adeco_amd.lattice import make_snapshot, write_sorted_by_id
+    from adeco_amd.workflow import AMD_Parameters, deposit_workflow
+    t0 = time.perf_counter()
+    snap = make_snapshot(n, RMS, seed=SEED)
+    log(f"[program] snapshot {n}^3 generated in {time.perf_counter() - t0:.1f}s")
+    out = {"particles": n ** 3, "tets": 6 * (n - 1) ** 3, "runs": []}
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "snapshot.bin")
+        write_sorted_by_id(path, snap)
+        p = AMD_Parameters(input_file=path, id_type="SORTED_BY_ID", max_refinement_level=MAX_LEVEL,
+                           max_tets_per_octree_node=MAX_TETS, linear_patch_resolution=DIMS,
+                           periodic_boundaries=True)
+        for r in range(runs + 1):
+            p.output_directory = os.path.join(tmp, f"out{r}")
+            os.mkdir(p.output_directory)
+            t0 = time.perf_counter()
+            res = deposit_workflow(p)
+            wall = time.perf_counter() - t0
+            rec = {"wall_s": wall, **{k: res[k] for k in ("load_s", "octree_s", "deposit_s", "write_s", "kernel_ms")},
+                   "leaves": res["num_leaves"], "total_mass": res["total_mass"]}
+            log(f"[program] run {r}: {wall * 1e3:.1f} ms (load {res['load_s'] * 1e3:.1f}, octree "
+                f"{res['octree_s'] * 1e3:.1f}, deposit {res['deposit_s'] * 1e3:.1f} [kernel {res['kernel_ms']:.2f}], "
+                f"files {res['write_s'] * 1e3:.1f}); {res['num_leaves']} leaves, mass {res['total_mass']:.6f}")
+            if r:
+                out["runs"].append(rec)
+        if with_reference:
+            out["cpu_baseline"] = reference_program_time(path, p, n, tmp)
+    w = float(np.median([r["wall_s"] for r in out["runs"]]))
+    out["median_wall_s"] = w
+    out["tets_per_s_end_to_end"] = out["tets"] / w
+    if out.get("cpu_baseline", {}).get("value"):
+        out["cpu_baseline"]["gpu_speedup"] = out["cpu_baseline"]["wall_s"] / w
+    return out
+
+
+def reference_program_time(snapshot_path, params, params_n, tmp):
+    """CPU baseline of the program study: the reference program itself (main.cpp, compiled from
+    the reference sources into oracle/_ref by oracle/build_ref.sh) on the same snapshot and
+    parameters, with one worker thread per core (at most 16), in a child process (it finalises
+    MPI).  Wall time of the child, which includes process and MPI start-up."""
+    import subprocess
+    ref = os.path.join(ROOT, "oracle", "_ref", "libadeco_ref.so")
+    if not os.path.exists(ref):
+        return {"value": None, "note": "oracle/_ref not built"}
+    threads = min(16, os.cpu_count() or 1)
+    outdir = os.path.join(tmp, "ref_out")
+    os.mkdir(outdir)
+    pfile = os.path.join(tmp, "ref_params.txt")
+    with open(pfile, "w") as f:
+        f.write(f"input_file               ={snapshot_path}\noutput_directory         ={outdir}\n"
+                f"file_format              = SORTED_BY_ID\nuse_region_of_interest   = false\n"
+                f"max_refinement_level     = {params.max_refinement_level}\nstride                   = 1\n"
+                f"max_tets_per_octree_node = {params.max_tets_per_octree_node}\n"
+                f"linear_patch_resolution  = {params.linear_patch_resolution}\n"
+                f"num_mpi_ranks_per_node   = 1\nnum_pthreads             = {threads}\n"
+                f"periodic_boundaries      = true\n")
+    code = ("import sys; sys.path.insert(0, sys.argv[2]); from tests import _oracle as O; "
+            "sys.exit(O.load_ref().ref_cli_main(sys.argv[1].encode()))")
+    log(f"[program] reference program on {threads} threads")
+    t0 = time.perf_counter()
+    r = subprocess.run([sys.executable, "-c", code, pfile, ROOT], capture_output=True, text=True, timeout=900)
+    wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        return {"value": None, "note": f"reference program failed: {r.stderr[-300:]}"}
+    total = None
+    for line in r.stdout.splitlines():
+        if "total mass" in line:
+            total = float(line.split("=")[-1])
+    files = len(os.listdir(outdir))
+    log(f"[program] reference program: {wall:.2f} s, {files} leaf files, total mass {total}")
+    tets = 6 * (params_n - 1) ** 3
+    return {"value": tets / wall, "unit": "tets/s", "cores": threads, "wall_s": wall, "leaf_files": files,
+            "total_mass": total, "kind": "reference", "sample": "the whole program on the same snapshot"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,6 +448,8 @@ def main():
     ap.add_argument("--serve", action="store_true", help="run only the ROI serving study (256^3 lattice)")
     ap.add_argument("--precision", action="store_true", help="run only the precision-mode study")
     ap.add_argument("--heavy-tail", action="store_true", help="run only the 256^3 heavy-tailed batched study")
+    ap.add_argument("--program", action="store_true",
+                    help="run only the whole single-rank program (file in, leaf files out) on a 128^3 snapshot")
     ap.add_argument("--serve-rates", default="20,50,100,200")
     ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
@@ -375,6 +461,12 @@ def main():
         print(json.dumps({"metric": "tets/s exact deposit, batched vs plain per-leaf launches", "unit": "tets/s",
                           "higher_is_better": True, "n_gpus": 1, "dtype": "f64",
                           "data": "synthetic (256^3 lattice, rms 4 spacings + 64 infall centres)", **res}))
+        return
+    if args.program:
+        res = program_study(with_reference=not args.no_cpu_baseline)
+        print(json.dumps({"metric": "tets/s end to end: snapshot file to leaf files", "unit": "tets/s",
+                          "value": res["tets_per_s_end_to_end"], "higher_is_better": True, "n_gpus": 1,
+                          "dtype": "f64", "data": "synthetic (128^3 periodic snapshot, rms 3 spacings)", **res}))
         return
     if args.precision:
         res = precision_study()
