@@ -83,7 +83,7 @@ int main(int argc, char** argv) {
     std::printf("INFO: RESULT: deposit_leaf_nodes: total mass                  =  %.17g\n", r.total_mass);
     std::printf("INFO: RESULTS: time for loading data          == %g\n", r.load_s);
     std::printf("INFO: RESULTS: time for constructing octree   == %g\n", r.octree_s);
-    std::printf("INFO: RESULTS: time for mass deposit          == %g (kernel %g ms)\n", r.deposit_s, r.kernel_ms);
+    std::printf("INFO: RESULTS: time for mass deposit          == %g (gpu %g ms)\n", r.deposit_s, r.kernel_ms);
     std::printf("INFO: RESULTS: time for writing leaf files    == %g\n", r.write_s);
     std::printf("INFO: (MPI rank = 0): finished successfully. \n\n");
     return 0;

@@ -383,7 +383,7 @@ def program_study(n=N, runs=3, with_reference=True):
             rec = {"wall_s": wall, **{k: res[k] for k in ("load_s", "octree_s", "deposit_s", "write_s", "kernel_ms")},
                    "leaves": res["num_leaves"], "total_mass": res["total_mass"]}
             log(f"[program] run {r}: {wall * 1e3:.1f} ms (load {res['load_s'] * 1e3:.1f}, octree "
-                f"{res['octree_s'] * 1e3:.1f}, deposit {res['deposit_s'] * 1e3:.1f} [kernel {res['kernel_ms']:.2f}], "
+                f"{res['octree_s'] * 1e3:.1f}, deposit {res['deposit_s'] * 1e3:.1f} [gpu {res['kernel_ms']:.2f}], "
                 f"files {res['write_s'] * 1e3:.1f}); {res['num_leaves']} leaves, mass {res['total_mass']:.6f}")
             if r:
                 out["runs"].append(rec)

@@ -259,7 +259,7 @@ typedef struct {
     int64_t num_nodes, num_leaves;
     double total_mass;     /* sum of the leaf files' masses in the leaf processing order */
     double load_s, octree_s, deposit_s, write_s; /* wall time of the steps */
-    double kernel_ms;      /* the leaf deposit's kernel time */
+    double kernel_ms;      /* the leaf deposit's GPU time (HIP events around its kernels) */
 } adeco_workflow_result;
 /* deposit_workflow (AdDepositWorkFlow.h:60-237) on one rank and one GPU: read the particles,
  * upload them, global bbox, periodic correction when periodic_boundaries is set, octree over the
