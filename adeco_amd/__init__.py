@@ -168,6 +168,23 @@ class ResidentLattice:
     def from_lattice(cls, lat, device: int = 0):
         return cls(lat.x, lat.y, lat.z, lat.dims, device)
 
+    @classmethod
+    def from_dark_sky(cls, filename: str, stride: int = 1, device: int = 0):
+        """A DARK_SKY particle file (AdDarkSkyDataReader, AdPointDataReaders.h:361-767) read straight
+        into device memory: each record's Morton id places its position on the lattice."""
+        self = cls.__new__(cls)
+        h = _lib.c_vp()
+        _check(load().adeco_lattice_create_dark_sky(int(device), os.fsencode(filename), int(stride),
+                                                    _lib.ctypes.byref(h)), "lattice_create_dark_sky()")
+        self.handle = h
+        self.device = device
+        n = _lib.ctypes.c_int64()
+        d = np.zeros(1, np.int32)
+        _check(load().adeco_dark_sky_info(os.fsencode(filename), None, _lib.ctypes.byref(n), ptr(d)), "dark_sky_info()")
+        side = int(d[0]) // int(stride)
+        self.dims = (side, side, side)
+        return self
+
     @property
     def num_tets(self) -> int:
         return 6 * (self.dims[0] - 1) * (self.dims[1] - 1) * (self.dims[2] - 1)

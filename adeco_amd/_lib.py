@@ -64,6 +64,9 @@ SIGNATURES = {
     "adeco_parse_parameter_file": (ctypes.c_int, [ctypes.c_char_p, c_vp]),
     "adeco_read_sorted_by_id": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, c_vp, c_vp, c_vp, ctypes.c_int64,
                                                c_i64p, c_vp]),
+    "adeco_lattice_create_dark_sky": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "adeco_dark_sky_info": (ctypes.c_int, [ctypes.c_char_p, c_i64p, c_i64p, c_vp]),
+    "adeco_selftest_dark_sky_site": (None, [ctypes.c_longlong, c_vp]),
     "adeco_workflow_options_init": (None, [c_vp]),
     "adeco_deposit_workflow": (ctypes.c_int, [c_vp, c_vp, c_vp]),
     "adeco_last_timings": (None, [c_vp]),

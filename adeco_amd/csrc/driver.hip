@@ -1,7 +1,7 @@
 // The program around the deposit on one rank (C ABI part 7): the parameter file
 // (parse_parameter_file, main.cpp:83-183), the SORTED_BY_ID particle reader
 // (AdSortedByIdsPointDataReader, AdPointDataReaders.h:770-911, with the checks of
-// AdPointDataReader::load_data, :231-276) and the workflow (deposit_workflow,
+// AdPointDataReader::load_data, :231-276; DARK_SKY files go to darksky.hip) and the workflow (deposit_workflow,
 // AdDepositWorkFlow.h:60-237) over the resident lattice: read, global bbox, periodic correction,
 // octree, leaf deposit, leaf files.  Host code; the compute runs in the HIP kernels behind the
 // other entry points.
@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/adeco.h"
+#include "lattice_impl.hpp"
 #include "runtime.hpp"
 
 using namespace adeco;
@@ -193,31 +194,30 @@ adeco_status adeco_deposit_workflow(const adeco_parameters* p, const adeco_workf
     adeco_workflow_result r;
     std::memset(&r, 0, sizeof(r));
     const std::string format(p->id_type);
-    if (format.find("SORTED_BY_ID") == std::string::npos) {
-        if (format.find("DARK_SKY") != std::string::npos)
-            return fail(ADECO_EINVAL, "ERROR: id_type " + format + ": the DARK_SKY reader is not part of this library.");
+    const bool sorted = format.find("SORTED_BY_ID") != std::string::npos;
+    if (!sorted && format.find("DARK_SKY") == std::string::npos)
         return fail(ADECO_EINVAL, "ERROR: id_type " + format + " not supported.");
-    }
     // load (AdDepositWorkFlow.h:77-97)
     auto t0 = std::chrono::steady_clock::now();
-    int64_t n = 0;
-    int d[3];
-    adeco_status st = adeco_read_sorted_by_id(p->input_file, p->stride, nullptr, nullptr, nullptr, 0, &n, d);
-    if (st) return st;
-    const size_t count = static_cast<size_t>(n);
-    std::vector<float> x(count), y(count), z(count);
-    if ((st = adeco_read_sorted_by_id(p->input_file, p->stride, x.data(), y.data(), z.data(), n, &n, d))) return st;
-    r.num_particles = n;
-    std::memcpy(r.particle_dims, d, sizeof(d));
     adeco_lattice L = nullptr;
-    if ((st = adeco_lattice_create(opt->device, x.data(), y.data(), z.data(), d[0], d[1], d[2], &L))) return st;
+    adeco_status st = ADECO_OK;
+    if (sorted) {
+        int64_t n = 0;
+        int d[3];
+        if ((st = adeco_read_sorted_by_id(p->input_file, p->stride, nullptr, nullptr, nullptr, 0, &n, d))) return st;
+        const size_t count = static_cast<size_t>(n);
+        std::vector<float> x(count), y(count), z(count);
+        if ((st = adeco_read_sorted_by_id(p->input_file, p->stride, x.data(), y.data(), z.data(), n, &n, d))) return st;
+        if ((st = adeco_lattice_create(opt->device, x.data(), y.data(), z.data(), d[0], d[1], d[2], &L))) return st;
+    } else if ((st = adeco_lattice_create_dark_sky(opt->device, p->input_file, p->stride, &L))) {
+        return st;
+    }
     struct LatticeGuard {
         adeco_lattice L;
         ~LatticeGuard() { adeco_lattice_destroy(L); }
     } lg{L};
-    std::vector<float>().swap(x);
-    std::vector<float>().swap(y);
-    std::vector<float>().swap(z);
+    for (int a = 0; a < 3; ++a) r.particle_dims[a] = L->np[a];
+    r.num_particles = L->n_vertices;
     r.load_s = seconds_since(t0);
     // global bbox, periodic correction, region of interest (AdDepositWorkFlow.h:101-115)
     t0 = std::chrono::steady_clock::now();

@@ -154,6 +154,35 @@ adeco_status lattice_bbox_device(adeco_lattice_s* L, float out6[6]) {
     return ADECO_OK;
 }
 
+// A lattice handle with uninitialised device arrays for npx*npy*npz vertices on device_id.
+adeco_status lattice_alloc(int device_id, int npx, int npy, int npz, adeco_lattice_s*& out) {
+    out = nullptr;
+    adeco_status st = check_device(device_id);
+    if (st) return st;
+    DeviceState& ds = device_state(device_id);
+    std::lock_guard<std::mutex> lk(ds.mutex);
+    if ((st = ds.ensure(device_id))) return st;
+    hipError_t e = hipSetDevice(device_id);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    auto* L = new adeco_lattice_s();
+    L->device = device_id;
+    L->ds = &ds;
+    L->np[0] = npx;
+    L->np[1] = npy;
+    L->np[2] = npz;
+    L->n_vertices = int64_t(npx) * npy * npz;
+    const size_t bytes = size_t(L->n_vertices) * sizeof(float);
+    e = hipMalloc(&L->d_x, bytes);
+    if (e == hipSuccess) e = hipMalloc(&L->d_y, bytes);
+    if (e == hipSuccess) e = hipMalloc(&L->d_z, bytes);
+    if (e != hipSuccess) {
+        delete L;
+        return fail(ADECO_ENOMEM, std::string("lattice_create(): ") + hipGetErrorString(e));
+    }
+    out = L;
+    return ADECO_OK;
+}
+
 }  // namespace adeco
 
 using namespace adeco;
@@ -166,32 +195,17 @@ adeco_status adeco_lattice_create(int device_id, const float* x, const float* y,
     *out = nullptr;
     if (npx < 2 || npy < 2 || npz < 2) return fail(ADECO_EINVAL, "lattice_create(): need at least 2 vertices per axis");
     const int64_t n = int64_t(npx) * npy * npz;
-    if (n > (int64_t(1) << 31) / 6)  // tet ids are int32 in the binned work lists
-        return fail(ADECO_EOVERFLOW, "lattice_create(): lattice too large for 32-bit tet ids");
+    if (n > kMaxLatticeVertices) return fail(ADECO_EOVERFLOW, "lattice_create(): lattice too large for 32-bit tet ids");
     for (int64_t i = 0; i < n; ++i)
         if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]))
             return fail(ADECO_EINVAL, "lattice_create(): non-finite vertex coordinate");
-    adeco_status st = check_device(device_id);
+    adeco_lattice_s* L = nullptr;
+    adeco_status st = lattice_alloc(device_id, npx, npy, npz, L);
     if (st) return st;
-    DeviceState& ds = device_state(device_id);
+    DeviceState& ds = *L->ds;
     std::lock_guard<std::mutex> lk(ds.mutex);
-    if ((st = ds.ensure(device_id))) return st;
-    hipError_t e = hipSetDevice(device_id);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    auto* L = new adeco_lattice_s();
-    L->device = device_id;
-    L->ds = &ds;
-    L->np[0] = npx; L->np[1] = npy; L->np[2] = npz;
-    L->n_vertices = n;
     const size_t bytes = size_t(n) * sizeof(float);
-    e = hipMalloc(&L->d_x, bytes);
-    if (e == hipSuccess) e = hipMalloc(&L->d_y, bytes);
-    if (e == hipSuccess) e = hipMalloc(&L->d_z, bytes);
-    if (e != hipSuccess) {
-        delete L;
-        return fail(ADECO_ENOMEM, std::string("lattice_create(): ") + hipGetErrorString(e));
-    }
-    e = hipMemcpyAsync(L->d_x, x, bytes, hipMemcpyHostToDevice, ds.stream);
+    hipError_t e = hipMemcpyAsync(L->d_x, x, bytes, hipMemcpyHostToDevice, ds.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(L->d_y, y, bytes, hipMemcpyHostToDevice, ds.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(L->d_z, z, bytes, hipMemcpyHostToDevice, ds.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);

@@ -32,5 +32,8 @@ struct adeco_lattice_s {
 };
 
 namespace adeco {
+// tet ids are int32 in the binned work lists: at most 2^31 / 6 vertices
+constexpr int64_t kMaxLatticeVertices = (int64_t(1) << 31) / 6;
 adeco_status lattice_bbox_device(adeco_lattice_s* L, float out6[6]);
+adeco_status lattice_alloc(int device_id, int npx, int npy, int npz, adeco_lattice_s*& out);
 }

@@ -118,6 +118,41 @@ def write_sorted_by_id(path: str, lat: Lattice) -> None:
     np.stack([lat.x, lat.y, lat.z], axis=1).astype(np.float32).tofile(path)
 
 
+def morton_ids(i, j, k) -> np.ndarray:
+    """DARK_SKY particle ids: the Morton code of lattice site (i, j, k), x in bits 0, 3, 6, ...,
+    y in bits 1, 4, ..., z in bits 2, 5, ... (AdPointDataReaders.h:366-414)."""
+    i, j, k = (np.asarray(a, np.uint64) for a in (i, j, k))
+    out = np.zeros(np.broadcast(i, j, k).shape, np.uint64)
+    for b in range(16):
+        bit = np.uint64(1 << b)
+        out |= ((i & bit) << np.uint64(2 * b)) | ((j & bit) << np.uint64(2 * b + 1)) | ((k & bit) << np.uint64(2 * b + 2))
+    return out.astype(np.int64)
+
+
+def write_dark_sky(path: str, lat: Lattice, seed: int = 7, sha1_chunks: int = 2) -> None:
+    """Writes a lattice as a DARK_SKY (SDF) file: a text header ending with "# SDF-EOH",
+    `sha1_chunks` records of 24 bytes, then one record per particle {float x, y, z, vx, vy, vz;
+    int64 id} in a seeded random order, id = the Morton code of the particle's site, velocities
+    seeded noise (AdPointDataReaders.h:361-767)."""
+    nx, ny, nz = lat.dims
+    rng = np.random.default_rng(seed)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    rec = np.zeros(lat.x.size, dtype=[("pos", np.float32, 3), ("vel", np.float32, 3), ("id", np.int64)])
+    rec["pos"] = np.stack([lat.x, lat.y, lat.z], axis=1)
+    rec["vel"] = rng.normal(0.0, 100.0, (lat.x.size, 3)).astype(np.float32)
+    rec["id"] = morton_ids(i.ravel(), j.ravel(), k.ravel())
+    rec = rec[rng.permutation(rec.size)]
+    header = ("# SDF 1.0\n"
+              "parameter byteorder = 0x78563412;\n"
+              f"int sha1_chunks = {sha1_chunks};\n"
+              "struct {\n  float x, y, z;\n  float vx, vy, vz;\n  int64_t ident;\n}[" + str(rec.size) + "];\n"
+              "# SDF-EOH\n")
+    with open(path, "wb") as f:
+        f.write(header.encode())
+        f.write(rng.integers(0, 256, 24 * sha1_chunks, dtype=np.uint8).tobytes())
+        f.write(rec.tobytes())
+
+
 def _apply_infall(px, py, pz, box, count, rng):
     """Spherical infall: x -> x - A (x - c) w(|x - c| / R) with w(r) = (1 - r^2)^2 for r < 1.
 

@@ -3,7 +3,8 @@
 Mirrors, by name and argument meaning:
 
 * ``AMD_Parameters`` and ``parse_parameter_file`` (AdTypeDefinitions.h:1218-1247, main.cpp:83-183);
-* ``get_reader`` / ``AdSortedByIdsPointDataReader.load_data`` (AdPointDataReaders.h:770-930);
+* ``get_reader``, ``AdSortedByIdsPointDataReader`` and ``AdDarkSkyDataReader``
+  (AdPointDataReaders.h:361-930);
 * ``deposit_workflow`` (AdDepositWorkFlow.h:60-237): read, global bbox, periodic correction,
   octree, leaf deposit, one ``node_<level>_<ix>_<iy>_<iz>`` density file per leaf.
 
@@ -99,13 +100,37 @@ class AdSortedByIdsPointDataReader:
         return x, y, z, tuple(int(d) for d in dims)
 
 
+class AdDarkSkyDataReader:
+    """DARK_SKY particles (AdPointDataReaders.h:361-767): a text header, then 32-byte records
+    {float x, y, z, vx, vy, vz; int64 id} whose Morton ids place them on a cubic lattice.  The
+    records are decoded and placed on the GPU."""
+
+    def info(self, filename: str) -> dict:
+        """Header bytes, particle records and particles per axis (host-only checks)."""
+        h, n = ctypes.c_int64(), ctypes.c_int64()
+        d = np.zeros(1, np.int32)
+        _check(load().adeco_dark_sky_info(os.fsencode(filename), ctypes.byref(h), ctypes.byref(n), ptr(d)),
+               "dark_sky_info()")
+        return {"header_bytes": h.value, "num_particles": n.value, "dims": int(d[0])}
+
+    def load_lattice(self, filename: str, stride: int = 1, device: int = 0):
+        """The particles as a ResidentLattice on the GPU."""
+        from . import ResidentLattice
+        return ResidentLattice.from_dark_sky(filename, stride, device)
+
+    def load_data(self, filename: str, stride: int = 1, device: int = 0):
+        """Returns (x, y, z, dims) as AdSortedByIdsPointDataReader.load_data does."""
+        with self.load_lattice(filename, stride, device) as rl:
+            x, y, z = rl.positions()
+            return x, y, z, rl.dims
+
+
 def get_reader(format: str):
-    """get_reader (AdPointDataReaders.h:914-930): SORTED_BY_ID only (the DARK_SKY reader is not
-    part of this library)."""
+    """get_reader (AdPointDataReaders.h:914-930): SORTED_BY_ID, then DARK_SKY, by substring."""
     if "SORTED_BY_ID" in format:
         return AdSortedByIdsPointDataReader()
     if "DARK_SKY" in format:
-        raise AdRuntimeException(ADECO_EINVAL, f"ERROR: id_type {format}: the DARK_SKY reader is not part of this library.")
+        return AdDarkSkyDataReader()
     raise AdRuntimeException(ADECO_EINVAL, f"ERROR: id_type {format} not supported.")
 
 

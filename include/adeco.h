@@ -19,6 +19,7 @@
  *                            + gather_local_tets                     AdDynamicLoadBalancing.h:894-950
  *   adeco_parse_parameter_file  parse_parameter_file                 main.cpp:83-183
  *   adeco_read_sorted_by_id  AdSortedByIdsPointDataReader::load_data AdPointDataReaders.h:770-911
+ *   adeco_lattice_create_dark_sky  AdDarkSkyDataReader::load_data  AdPointDataReaders.h:361-767
  *   adeco_deposit_workflow   deposit_workflow (one rank)             AdDepositWorkFlow.h:60-237
  *   adeco_write_patch        AdOctreeGridWriter::write_patch         AdOctreeGridWriter.h:67-144
  *   adeco_write_leaves       the write step of deposit_leaf_nodes    AdDynamicLoadBalancing.h:998,1133,1303-1312
@@ -244,6 +245,19 @@ adeco_status adeco_parse_parameter_file(const char* filename, adeco_parameters* 
  * otherwise ignored, as in the reference. */
 adeco_status adeco_read_sorted_by_id(const char* filename, int stride, float* x, float* y, float* z, int64_t capacity,
                                      int64_t* num_particles, int* dims3);
+/* DARK_SKY particles (AdDarkSkyDataReader, AdPointDataReaders.h:361-767) on one rank, straight
+ * into a resident lattice: a text header ending with a "# SDF-EOH" line, "int sha1_chunks = N;"
+ * records of 24 bytes after it, then 32-byte particle records {float x,y,z,vx,vy,vz; int64 id},
+ * the id's low 48 bits the Morton code of the lattice site (x in bits 0,3,6,..., y in 1,4,...,
+ * z in 2,5,...).  n^3 records; with stride s (dividing n) the sites whose indices are multiples of
+ * s, at site / s.  The records stream to the device, which decodes the ids and stores each
+ * position at its site; an id outside the lattice, a repeated site or a missing one is
+ * ADECO_EINVAL (the reference's checks and assertions). */
+adeco_status adeco_lattice_create_dark_sky(int device_id, const char* filename, int stride, adeco_lattice* out);
+/* Host-only: the header and size checks of a DARK_SKY file (get_header_size,
+ * AdPointDataReaders.h:419-466, and the record count checks): header bytes, particle records and
+ * particles per axis before any stride. */
+adeco_status adeco_dark_sky_info(const char* filename, int64_t* header_bytes, int64_t* num_particles, int* dims);
 typedef struct {
     int device;
     int mode;           /* ADECO_MODE_* (default batched) */
@@ -261,10 +275,11 @@ typedef struct {
     double load_s, octree_s, deposit_s, write_s; /* wall time of the steps */
     double kernel_ms;      /* the leaf deposit's GPU time (HIP events around its kernels) */
 } adeco_workflow_result;
-/* deposit_workflow (AdDepositWorkFlow.h:60-237) on one rank and one GPU: read the particles,
- * upload them, global bbox, periodic correction when periodic_boundaries is set, octree over the
- * region of interest (the global bbox unless use_regions_of_interest), leaf deposit at
- * linear_patch_resolution and one write_patch file per leaf into output_directory. */
+/* deposit_workflow (AdDepositWorkFlow.h:60-237) on one rank and one GPU: read the particles
+ * (id_type SORTED_BY_ID or DARK_SKY, chosen as get_reader does, AdPointDataReaders.h:914-930)
+ * into a resident lattice, global bbox, periodic correction when periodic_boundaries is set,
+ * octree over the region of interest (the global bbox unless use_regions_of_interest), leaf
+ * deposit at linear_patch_resolution and one write_patch file per leaf into output_directory. */
 adeco_status adeco_deposit_workflow(const adeco_parameters* p, const adeco_workflow_options* opt,
                                     adeco_workflow_result* res);
 
@@ -331,6 +346,8 @@ void adeco_selftest_decomposition(int decomposition, int32_t* corners24);
  * cell c summed by the signed (use_signed != 0) or the exact tree, sab_out[c] = the magnitudes it
  * was summed from. */
 void adeco_selftest_column_f64(const double* tet12, int ls, int use_signed, double* vol6_out, double* sab_out);
+/* The DARK_SKY id decoding the reader's kernel uses: lattice site (x, y, z) of an id. */
+void adeco_selftest_dark_sky_site(long long id, int* site3);
 /* The kernels' per-tet FP64 deposit routine run on the host over a tet soup (float [n][4][3]),
  * accumulating into grid (double, dims^3). */
 adeco_status adeco_selftest_deposit_soup_f64(const float* tets, int64_t n_tets, const float* bbox6, int dims,

@@ -19,6 +19,8 @@
 //                            start-up, deposit_workflow (AdDepositWorkFlow.h:60-237), MPI_Finalize
 //   ref_apply_boundary_corrections -> apply_boundary_corrections (AdPointDataReaders.h:156-206)
 //   ref_sorted_by_id_load -> AdSortedByIdsPointDataReader::load_data (AdPointDataReaders.h:770-911)
+//   ref_dark_sky_load     -> AdDarkSkyDataReader::load_data (AdPointDataReaders.h:361-767)
+//   ref_dark_sky_header_size -> AdDarkSkyDataReader::get_header_size (AdPointDataReaders.h:419-466)
 
 #include <mpi.h>
 #include <fstream>
@@ -48,6 +50,11 @@ void parse_parameter_file(const std::string& filename, AMD_Parameters& parameter
 namespace {
 
 std::string g_err;
+
+// exposes the reader's protected header parser
+struct DarkSkyHeader : AdDarkSkyDataReader {
+    static size_t size(const std::string& f) { return get_header_size(f); }
+};
 
 void ensure_mpi() {
     int flag = 0;
@@ -325,6 +332,47 @@ int64_t ref_sorted_by_id_load(const char* path, int64_t cap, float* x, float* y,
             dims3[2] = d[2];
         }
         return n;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Loads a DARK_SKY file on one rank with the given stride.  Returns the particle count (-1 on
+// error); with cap at least that count, x, y, z and ids receive the particles in load order (ids:
+// the linear lattice index after the stride) and dims3 the lattice.
+int64_t ref_dark_sky_load(const char* path, int stride, int64_t cap, float* x, float* y, float* z, int64_t* ids,
+                          int* dims3) {
+    try {
+        ensure_mpi();
+        std::shared_ptr<AdPointDataReader> reader = get_reader("DARK_SKY");
+        reader->load_data(path, 0, 1, stride);
+        const Particles& p = *reader->get_particles_per_rank()[0];
+        const int64_t n = int64_t(p.get_num());
+        if (n <= cap) {
+            for (int64_t i = 0; i < n; ++i) {
+                pos_t pos[3];
+                p.get_position(size_t(i), pos);
+                x[i] = pos[0];
+                y[i] = pos[1];
+                z[i] = pos[2];
+                ids[i] = int64_t(p.get_id(size_t(i)));
+            }
+            const AvVec3i d = reader->get_global_particle_dims();
+            dims3[0] = d[0];
+            dims3[1] = d[1];
+            dims3[2] = d[2];
+        }
+        return n;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int64_t ref_dark_sky_header_size(const char* path) {
+    try {
+        return int64_t(DarkSkyHeader::size(path));
     } catch (std::exception& e) {
         g_err = e.what();
         return -1;

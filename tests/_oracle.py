@@ -93,6 +93,8 @@ _REF_SIGS = {
     "ref_parse_parameter_file": (cint, [ctypes.c_char_p, vp, cint, vp, vp]),
     "ref_apply_boundary_corrections": (cint, [vp, vp, vp, cint, cint, cint, vp]),
     "ref_sorted_by_id_load": (i64, [ctypes.c_char_p, i64, vp, vp, vp, vp, vp]),
+    "ref_dark_sky_load": (i64, [ctypes.c_char_p, cint, i64, vp, vp, vp, vp, vp]),
+    "ref_dark_sky_header_size": (i64, [ctypes.c_char_p]),
 }
 
 _cache = {}

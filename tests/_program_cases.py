@@ -1,5 +1,5 @@
-"""Inputs of the program-level golden cases (checker side): parameter files, SORTED_BY_ID
-snapshots and whole runs of the reference program.  tests/golden/make_program_golden.py runs the
+"""Inputs of the program-level golden cases (checker side): parameter files, SORTED_BY_ID and
+DARK_SKY snapshots and whole runs of the reference program.  tests/golden/make_program_golden.py runs the
 compiled reference on them; the tests run this library and the oracle on the same inputs.
 """
 
@@ -54,6 +54,15 @@ PROGRAM_CASES = {
                    "stride": "1", "max_tets_per_octree_node": "4000", "linear_patch_resolution": "8",
                    "num_mpi_ranks_per_node": "1", "num_pthreads": "1", "periodic_boundaries": "true"},
     },
+    # the periodic case's snapshot as a DARK_SKY file (shuffled records, Morton ids): the same
+    # lattice, so the same leaf files
+    "dark_sky_periodic": {
+        "snapshot": (24, 2.0, 3),
+        "format": "DARK_SKY",
+        "params": {"file_format": "DARK_SKY", "use_region_of_interest": "false", "max_refinement_level": "4",
+                   "stride": "1", "max_tets_per_octree_node": "4000", "linear_patch_resolution": "8",
+                   "num_mpi_ranks_per_node": "1", "num_pthreads": "1", "periodic_boundaries": "true"},
+    },
     "roi_not_periodic": {
         "snapshot": (20, 1.5, 5),
         "params": {"file_format": "SORTED_BY_ID", "use_region_of_interest": "true",
@@ -61,6 +70,24 @@ PROGRAM_CASES = {
                    "max_tets_per_octree_node": "2500", "linear_patch_resolution": "12",
                    "num_mpi_ranks_per_node": "1", "num_pthreads": "1", "periodic_boundaries": "false"},
     },
+}
+
+
+def write_case_snapshot(case: dict, path: str, lat) -> None:
+    """Writes a case's snapshot in its file format (SORTED_BY_ID unless stated)."""
+    from adeco_amd.lattice import write_dark_sky, write_sorted_by_id
+    if case.get("format") == "DARK_SKY":
+        write_dark_sky(path, lat)
+    else:
+        write_sorted_by_id(path, lat)
+
+
+# DARK_SKY headers for the header parser: (text, sha1 records to append)
+DARK_SKY_HEADERS = {
+    "valid": ("# SDF 1.0\nint sha1_chunks = 3;\nstruct {\n  float x;\n}[8];\n# SDF-EOH\n", 3),
+    "no_semicolon": ("int sha1_chunks = 1\n# SDF-EOH more text\n", 1),
+    "no_end_of_header": ("# SDF 1.0\nint sha1_chunks = 2;\n", 0),
+    "no_sha1_chunks": ("# SDF 1.0\n# SDF-EOH\n", 0),
 }
 
 

@@ -6,7 +6,9 @@ ref_program.json next to this script:
   params    parse_parameter_file (main.cpp:83-183) on the texts of tests/_program_cases.py
   reader    AdSortedByIdsPointDataReader::load_data on a small valid file and on invalid ones
   unwrap    apply_boundary_corrections on a non-cubic wrapped lattice (sha256 of x|y|z after)
-  program   the reference program (main.cpp:187-249) run on SORTED_BY_ID snapshots: its leaf file
+  dark_sky  AdDarkSkyDataReader's header sizes and errors, and its loads of a shuffled DARK_SKY file
+            at several strides (the positions in id order: the one-rank redistribution's lattice)
+  program   the reference program (main.cpp:187-249) run on SORTED_BY_ID and DARK_SKY snapshots: its leaf file
             names, their sha256, the total mass it prints and the sha256 of the corrected
             coordinates (apply_boundary_corrections with the snapshot's bbox)
 
@@ -30,9 +32,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from adeco_amd.lattice import make_snapshot, write_sorted_by_id  # noqa: E402
+from adeco_amd.lattice import make_snapshot, write_dark_sky, write_sorted_by_id  # noqa: E402
 from tests import _oracle as O  # noqa: E402
-from tests._program_cases import PARAM_TEXTS, PROGRAM_CASES, param_file_text, unwrap_lattice_case  # noqa: E402
+from tests._program_cases import (DARK_SKY_HEADERS, PARAM_TEXTS, PROGRAM_CASES, param_file_text,  # noqa: E402
+                                  unwrap_lattice_case, write_case_snapshot)
 
 
 def sha(a) -> str:
@@ -99,6 +102,34 @@ def unwrap_fixture(r):
     return {"dims": list(dims), "bbox": bbox.tolist(), "xyz_sha": sha(np.concatenate([x, y, z]))}
 
 
+def dark_sky_fixture(r, tmp):
+    headers = {}
+    for name, (text, chunks) in DARK_SKY_HEADERS.items():
+        path = os.path.join(tmp, f"hdr_{name}.bin")
+        with open(path, "wb") as f:
+            f.write(text.encode() + bytes(24 * chunks) + bytes(32 * 8))
+        h = r.ref_dark_sky_header_size(path.encode())
+        headers[name] = {"header_bytes": int(h)} if h >= 0 else {"error": r.ref_last_error().decode().replace(tmp, "<dir>")}
+    lat = make_snapshot(12, 2.0, seed=3)
+    path = os.path.join(tmp, "ds12.bin")
+    write_dark_sky(path, lat)
+    loads = {}
+    for stride in (1, 2, 3, 5):
+        cap = 4096
+        x, y, z = (np.zeros(cap, np.float32) for _ in range(3))
+        ids = np.zeros(cap, np.int64)
+        dims = np.zeros(3, np.int32)
+        n = r.ref_dark_sky_load(path.encode(), stride, cap, O.P(x), O.P(y), O.P(z), O.P(ids), O.P(dims))
+        if n < 0:
+            loads[str(stride)] = {"error": r.ref_last_error().decode()}
+            continue
+        # the one-rank redistribution sorts the particles by id: the lattice in id order
+        order = np.argsort(ids[:n], kind="stable")
+        loads[str(stride)] = {"n": int(n), "dims": dims.tolist(), "ids_unique": bool(np.unique(ids[:n]).size == n),
+                              "xyz_sha": sha(np.concatenate([x[:n][order], y[:n][order], z[:n][order]]))}
+    return {"headers": headers, "snapshot": [12, 2.0, 3], "loads": loads}
+
+
 def program_fixture(r, tmp):
     out = {}
     for name, case in PROGRAM_CASES.items():
@@ -106,7 +137,7 @@ def program_fixture(r, tmp):
         d = os.path.join(tmp, name)
         os.makedirs(os.path.join(d, "out"))
         snap = os.path.join(d, "snap.bin")
-        write_sorted_by_id(snap, lat)
+        write_case_snapshot(case, snap, lat)
         pfile = os.path.join(d, "params.txt")
         with open(pfile, "w") as f:
             f.write(param_file_text(case, snap, os.path.join(d, "out")))
@@ -135,7 +166,7 @@ def main():
         sys.exit("oracle/_ref/libadeco_ref.so is not built (run oracle/build_ref.sh)")
     with tempfile.TemporaryDirectory() as tmp:
         fx = {"params": params_fixture(r, tmp), "reader": reader_fixture(r, tmp), "unwrap": unwrap_fixture(r),
-              "program": program_fixture(r, tmp)}
+              "dark_sky": dark_sky_fixture(r, tmp), "program": program_fixture(r, tmp)}
     with open(os.path.join(HERE, "ref_program.json"), "w") as f:
         json.dump(fx, f, indent=1, sort_keys=True)
     print("wrote ref_program.json")

@@ -20,12 +20,12 @@ import numpy as np
 import pytest
 
 import adeco_amd as A
-from adeco_amd.lattice import Lattice, make_snapshot, write_sorted_by_id
-from adeco_amd.workflow import deposit_workflow, parse_parameter_file
+from adeco_amd.lattice import Lattice, make_snapshot, morton_ids, write_dark_sky
+from adeco_amd.workflow import deposit_workflow, get_reader, parse_parameter_file
 from tests import _oracle as O
 from tests._golden_leaves import patch_file_name
 from tests._octree_io import oracle_octree
-from tests._program_cases import PROGRAM_CASES, param_file_text, unwrap_lattice_case
+from tests._program_cases import PROGRAM_CASES, param_file_text, unwrap_lattice_case, write_case_snapshot
 from tests._program_oracle import oracle_program, unwrapped
 
 pytestmark = pytest.mark.gpu
@@ -77,11 +77,58 @@ def test_gpu_octree_after_correction_is_rooted_at_the_global_bbox():
         assert np.array_equal(lv[k], want[k]), k
 
 
+def test_dark_sky_reader_matches_reference(tmp_path):
+    """The DARK_SKY file (shuffled records) read into the lattice on the GPU gives the reference
+    reader's particles in id order, bit for bit, at every stride; a stride that does not divide
+    the lattice fails with the reference's message."""
+    g = GOLDEN["dark_sky"]
+    lat = make_snapshot(*g["snapshot"][:2], seed=g["snapshot"][2])
+    path = tmp_path / "ds12.bin"
+    write_dark_sky(str(path), lat)
+    for stride, want in g["loads"].items():
+        print(f"[program] dark sky reader, stride {stride}", flush=True)
+        if "error" in want:
+            with pytest.raises(A.AdRuntimeException) as e:
+                A.ResidentLattice.from_dark_sky(str(path), int(stride))
+            assert want["error"].split("AdRuntimeException: ")[-1] in str(e.value)
+            continue
+        with A.ResidentLattice.from_dark_sky(str(path), int(stride)) as rl:
+            assert list(rl.dims) == want["dims"]
+            assert sha(np.concatenate(rl.positions())) == want["xyz_sha"]
+
+
+def test_dark_sky_reader_rejects_inconsistent_files(tmp_path):
+    """A repeated site, an id outside the lattice and a truncated record are errors (the
+    reference's count check and assertions), not silent garbage."""
+    lat = make_snapshot(8, 1.0, seed=1)
+    good = tmp_path / "good.bin"
+    write_dark_sky(str(good), lat)
+    raw = bytearray(good.read_bytes())
+    hdr = get_reader("DARK_SKY").info(str(good))["header_bytes"]
+    rec = np.frombuffer(raw, dtype=np.int64, offset=hdr).reshape(-1, 4)  # 4 words of 8 bytes per record
+    cases = {"repeated": (rec[1, 3], "inconsistent global particle number"),
+             "outside": (int(morton_ids(8, 0, 0)), "invalid dark sky id")}
+    for name, (new_id, msg) in cases.items():
+        bad = bytearray(raw)
+        np.frombuffer(bad, dtype=np.int64, offset=hdr).reshape(-1, 4)[0, 3] = new_id
+        path = tmp_path / f"{name}.bin"
+        path.write_bytes(bytes(bad))
+        print(f"[program] dark sky reader, {name} id", flush=True)
+        with pytest.raises(A.AdRuntimeException, match=msg):
+            A.ResidentLattice.from_dark_sky(str(path))
+    path = tmp_path / "short.bin"
+    path.write_bytes(bytes(raw[:-5]))
+    with pytest.raises(A.AdRuntimeException, match="inconsistent file size"):
+        A.ResidentLattice.from_dark_sky(str(path))
+    with A.ResidentLattice.from_dark_sky(str(good)) as rl:  # the reader still works after failures
+        assert np.array_equal(rl.positions()[0], lat.x)
+
+
 def _run_program(tmp_path, name, *flags):
     case = PROGRAM_CASES[name]
     lat = make_snapshot(*case["snapshot"][:2], seed=case["snapshot"][2])
     snap = tmp_path / "snap.bin"
-    write_sorted_by_id(str(snap), lat)
+    write_case_snapshot(case, str(snap), lat)
     out = tmp_path / "out"
     out.mkdir()
     pfile = tmp_path / "params.txt"

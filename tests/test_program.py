@@ -18,10 +18,11 @@ import numpy as np
 import pytest
 
 import adeco_amd as A
-from adeco_amd.lattice import make_snapshot, write_sorted_by_id
+from adeco_amd.lattice import make_snapshot, morton_ids, write_sorted_by_id
 from adeco_amd.workflow import AMD_Parameters, get_reader, parse_parameter_file
 from tests import _oracle as O
-from tests._program_cases import PARAM_TEXTS, PROGRAM_CASES, param_file_text, unwrap_lattice_case
+from tests._program_cases import (DARK_SKY_HEADERS, PARAM_TEXTS, PROGRAM_CASES, param_file_text,
+                                  unwrap_lattice_case)
 from tests._program_oracle import oracle_program, unwrapped
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -124,8 +125,39 @@ def test_sorted_by_id_reader_matches_reference(tmp_path):
         get_reader("SORTED_BY_ID").load_data(str(path), stride=0)
     with pytest.raises(A.AdRuntimeException, match="not supported"):
         get_reader("HDF5")
-    with pytest.raises(A.AdRuntimeException, match="DARK_SKY"):
-        get_reader("DARK_SKY")
+    assert type(get_reader(" DARK_SKY")).__name__ == "AdDarkSkyDataReader"
+
+
+@pytest.mark.parametrize("name", sorted(DARK_SKY_HEADERS))
+def test_dark_sky_header_matches_reference(tmp_path, name):
+    text, chunks = DARK_SKY_HEADERS[name]
+    path = tmp_path / f"hdr_{name}.bin"
+    path.write_bytes(text.encode() + bytes(24 * chunks) + bytes(32 * 8))
+    want = GOLDEN["dark_sky"]["headers"][name]
+    if "error" in want:
+        with pytest.raises(A.AdRuntimeException) as e:
+            get_reader("DARK_SKY").info(str(path))
+        assert ref_message(want["error"]).replace("<dir>", str(tmp_path)) in str(e.value)
+    else:
+        info = get_reader("DARK_SKY").info(str(path))
+        assert info == {"header_bytes": want["header_bytes"], "num_particles": 8, "dims": 2}
+
+
+def test_dark_sky_id_decoding():
+    """The reader kernel's id decoding (evaluated on the host) inverts the Morton codes the
+    reference's reader decodes (AdPointDataReaders.h:366-414): bit 3b of the id is bit b of x,
+    3b+1 of y, 3b+2 of z; bits above 47 are ignored."""
+    from adeco_amd._lib import load as lib_load
+    rng = np.random.default_rng(0)
+    ijk = rng.integers(0, 1 << 16, (2000, 3))
+    ids = morton_ids(ijk[:, 0], ijk[:, 1], ijk[:, 2])
+    site = np.zeros(3, np.int32)
+    for q in range(ids.size):
+        lib_load().adeco_selftest_dark_sky_site(int(ids[q]) | (0x7FFF << 48), O.P(site))
+        assert site.tolist() == ijk[q].tolist()
+    for b in range(48):  # one bit at a time
+        lib_load().adeco_selftest_dark_sky_site(1 << b, O.P(site))
+        assert site.tolist() == [(1 << (b // 3)) if a == b % 3 else 0 for a in range(3)]
 
 
 def test_oracle_periodic_correction_matches_reference():
