@@ -359,36 +359,44 @@ def program_study(n=N, runs=3, with_reference=True):
     3 spacings, wrapped into the box) read from a file, uploaded, periodic correction, octree
     (depth <= 6, split above 50,000 estimated tets), batched FP64 leaf deposit of the reference's
     tets (mass 1) at 16^3 cells per leaf, and one write_patch file per leaf.  One untimed run,
-    then `runs` timed runs; step times are the program's own.  with_reference: also time the
-    reference program on the same input (reference_program_time)."""
+    then `runs` timed runs; step times are the program's own.  The same snapshot as a DARK_SKY
+    file (shuffled 32-byte records, Morton ids placed on the GPU) is timed the same way.
+    with_reference: also time the reference program on the SORTED_BY_ID input
+    (reference_program_time)."""
     import tempfile
-    from adeco_amd.lattice import make_snapshot, write_sorted_by_id
+    from adeco_amd.lattice import make_snapshot, write_dark_sky, write_sorted_by_id
     from adeco_amd.workflow import AMD_Parameters, deposit_workflow
     t0 = time.perf_counter()
     snap = make_snapshot(n, RMS, seed=SEED)
     log(f"[program] snapshot {n}^3 generated in {time.perf_counter() - t0:.1f}s")
-    out = {"particles": n ** 3, "tets": 6 * (n - 1) ** 3, "runs": []}
+    out = {"particles": n ** 3, "tets": 6 * (n - 1) ** 3, "runs": [], "dark_sky_runs": []}
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "snapshot.bin")
         write_sorted_by_id(path, snap)
-        p = AMD_Parameters(input_file=path, id_type="SORTED_BY_ID", max_refinement_level=MAX_LEVEL,
-                           max_tets_per_octree_node=MAX_TETS, linear_patch_resolution=DIMS,
-                           periodic_boundaries=True)
-        for r in range(runs + 1):
-            p.output_directory = os.path.join(tmp, f"out{r}")
-            os.mkdir(p.output_directory)
-            t0 = time.perf_counter()
-            res = deposit_workflow(p)
-            wall = time.perf_counter() - t0
-            rec = {"wall_s": wall, **{k: res[k] for k in ("load_s", "octree_s", "deposit_s", "write_s", "kernel_ms")},
-                   "leaves": res["num_leaves"], "total_mass": res["total_mass"]}
-            log(f"[program] run {r}: {wall * 1e3:.1f} ms (load {res['load_s'] * 1e3:.1f}, octree "
-                f"{res['octree_s'] * 1e3:.1f}, deposit {res['deposit_s'] * 1e3:.1f} [gpu {res['kernel_ms']:.2f}], "
-                f"files {res['write_s'] * 1e3:.1f}); {res['num_leaves']} leaves, mass {res['total_mass']:.6f}")
-            if r:
-                out["runs"].append(rec)
+        ds_path = os.path.join(tmp, "snapshot.sdf")
+        write_dark_sky(ds_path, snap)
+        for fmt, fpath, key in (("SORTED_BY_ID", path, "runs"), ("DARK_SKY", ds_path, "dark_sky_runs")):
+            p = AMD_Parameters(input_file=fpath, id_type=fmt, max_refinement_level=MAX_LEVEL,
+                               max_tets_per_octree_node=MAX_TETS, linear_patch_resolution=DIMS,
+                               periodic_boundaries=True)
+            for r in range(runs + 1):
+                p.output_directory = os.path.join(tmp, f"out_{key}{r}")
+                os.mkdir(p.output_directory)
+                t0 = time.perf_counter()
+                res = deposit_workflow(p)
+                wall = time.perf_counter() - t0
+                rec = {"wall_s": wall,
+                       **{k: res[k] for k in ("load_s", "octree_s", "deposit_s", "write_s", "kernel_ms")},
+                       "leaves": res["num_leaves"], "total_mass": res["total_mass"]}
+                log(f"[program] {fmt} run {r}: {wall * 1e3:.1f} ms (load {res['load_s'] * 1e3:.1f}, octree "
+                    f"{res['octree_s'] * 1e3:.1f}, deposit {res['deposit_s'] * 1e3:.1f} [gpu {res['kernel_ms']:.2f}], "
+                    f"files {res['write_s'] * 1e3:.1f}); {res['num_leaves']} leaves, mass {res['total_mass']:.6f}")
+                if r:
+                    out[key].append(rec)
         if with_reference:
+            p.input_file, p.id_type = path, "SORTED_BY_ID"
             out["cpu_baseline"] = reference_program_time(path, p, n, tmp)
+    out["dark_sky_median_wall_s"] = float(np.median([r["wall_s"] for r in out["dark_sky_runs"]]))
     w = float(np.median([r["wall_s"] for r in out["runs"]]))
     out["median_wall_s"] = w
     out["tets_per_s_end_to_end"] = out["tets"] / w
