@@ -192,18 +192,33 @@ adeco_status adeco_lattice_create_dark_sky(int device_id, const char* filename, 
     DeviceState& ds = *L->ds;
     std::lock_guard<std::mutex> lk(ds.mutex);
     const int64_t sites = L->n_vertices;
-    constexpr int64_t kChunk = int64_t(1) << 20;  // records per transfer (32 MiB)
-    DevBuf d_seen, d_rec, d_cnt;
-    if ((st = d_seen.reserve(size_t(sites) * 4)) || (st = d_rec.reserve(size_t(kChunk) * sizeof(DarkSkyRecord))) ||
-        (st = d_cnt.reserve(32)))
+    // two staging buffers: the file read of one chunk overlaps the copy and scatter of the last
+    constexpr int64_t kChunk = int64_t(1) << 18;  // records per transfer (8 MiB)
+    constexpr size_t kChunkBytes = size_t(kChunk) * sizeof(DarkSkyRecord);
+    DevBuf d_seen, d_rec[2], d_cnt;
+    if ((st = d_seen.reserve(size_t(sites) * 4)) || (st = d_rec[0].reserve(kChunkBytes)) ||
+        (st = d_rec[1].reserve(kChunkBytes)) || (st = d_cnt.reserve(32)))
         return st;
-    DarkSkyRecord* h_rec = nullptr;
-    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&h_rec), size_t(kChunk) * sizeof(DarkSkyRecord));
-    if (e != hipSuccess) return hip_fail(e, "pinned staging buffer");
-    struct Pinned {
-        void* p;
-        ~Pinned() { (void)hipHostFree(p); }
-    } pinned{h_rec};
+    struct Staging {
+        DarkSkyRecord* h[2] = {nullptr, nullptr};
+        hipEvent_t copied[2] = {nullptr, nullptr};
+        hipStream_t stream = nullptr;
+        ~Staging() {
+            // nothing queued may still read or write these once they are released
+            if (stream) (void)hipStreamSynchronize(stream);
+            for (int b = 0; b < 2; ++b) {
+                if (h[b]) (void)hipHostFree(h[b]);
+                if (copied[b]) (void)hipEventDestroy(copied[b]);
+            }
+        }
+    } stg;
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+        e = hipHostMalloc(reinterpret_cast<void**>(&stg.h[b]), kChunkBytes);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&stg.copied[b], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) return hip_fail(e, "pinned staging buffers");
+    stg.stream = ds.stream;
     const unsigned nb_fill = unsigned(std::min<int64_t>(sites / 256 + 1, 4096));
     hipLaunchKernelGGL(k_fill_u32, dim3(nb_fill), dim3(256), 0, ds.stream, d_seen.as<unsigned>(), sites);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, ds.stream, d_cnt.as<unsigned>(), int64_t(8));
@@ -216,23 +231,27 @@ adeco_status adeco_lattice_create_dark_sky(int device_id, const char* filename, 
         ~Closer() { std::fclose(f); }
     } closer{f};
     if (std::fseek(f, long(header), SEEK_SET) != 0) return fail(ADECO_EIO, "lattice_create_dark_sky(): seek failed");
-    for (int64_t done = 0; done < n;) {
+    int64_t chunk = 0;
+    for (int64_t done = 0; done < n; ++chunk) {
+        const int b = int(chunk & 1);
         const int64_t take = std::min(kChunk, n - done);
-        // the previous chunk's kernel reads the device copy, not h_rec: the host buffer is free
-        // once its copy has completed, which the synchronisation below guarantees
-        if (std::fread(h_rec, sizeof(DarkSkyRecord), size_t(take), f) != size_t(take))
+        // host buffer b is free once the copy of two chunks ago has completed; device buffer b is
+        // reused in stream order, after that chunk's scatter
+        if (chunk >= 2 && (e = hipEventSynchronize(stg.copied[b])) != hipSuccess) return hip_fail(e, "dark sky staging");
+        if (std::fread(stg.h[b], sizeof(DarkSkyRecord), size_t(take), f) != size_t(take))
             return fail(ADECO_EIO, std::string("ERROR: AdDarkSkyDataReader::load_data_(): Failed to read all particles "
                                                "for lagrangian input grid: ") + filename);
-        e = hipMemcpyAsync(d_rec.ptr, h_rec, size_t(take) * sizeof(DarkSkyRecord), hipMemcpyHostToDevice, ds.stream);
+        e = hipMemcpyAsync(d_rec[b].ptr, stg.h[b], size_t(take) * sizeof(DarkSkyRecord), hipMemcpyHostToDevice,
+                           ds.stream);
+        if (e == hipSuccess) e = hipEventRecord(stg.copied[b], ds.stream);
         if (e != hipSuccess) return hip_fail(e, "dark sky upload");
         const unsigned nb = unsigned(std::min<int64_t>(take / 256 + 1, 4096));
-        hipLaunchKernelGGL(k_dark_sky_scatter, dim3(nb), dim3(256), 0, ds.stream, d_rec.as<DarkSkyRecord>(), take, full,
-                           stride, L->d_x, L->d_y, L->d_z, d_seen.as<unsigned>(), d_cnt.as<unsigned long long>());
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
-        if (e != hipSuccess) return hip_fail(e, "dark sky scatter");
+        hipLaunchKernelGGL(k_dark_sky_scatter, dim3(nb), dim3(256), 0, ds.stream, d_rec[b].as<DarkSkyRecord>(), take,
+                           full, stride, L->d_x, L->d_y, L->d_z, d_seen.as<unsigned>(), d_cnt.as<unsigned long long>());
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "dark sky scatter");
         done += take;
     }
+    if ((e = hipStreamSynchronize(ds.stream)) != hipSuccess) return hip_fail(e, "dark sky scatter");
     unsigned long long cnt[3] = {0, 0, 0};
     e = hipMemcpy(cnt, d_cnt.ptr, sizeof(cnt), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "dark sky counters");
