@@ -67,7 +67,7 @@ int main(int argc, char** argv) {
     adeco_parameters p;
     if (adeco_parse_parameter_file(argv[1], &p) != ADECO_OK) return error(adeco_last_error());
     print_parameters(p);
-    if (p.stride > 1)
+    if (p.stride > 1 && std::strstr(p.id_type, "SORTED_BY_ID"))
         std::printf("WARNING: AdSortedByIdsPointDataReader::load__data(): 'stride' parameter currently not supported - "
                     "will ignore it ... \n");
     if (adeco_set_cell_scaling(reference_geometry ? ADECO_SCALING_REFERENCE : ADECO_SCALING_EXACT) != ADECO_OK)
