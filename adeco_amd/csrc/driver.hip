@@ -40,7 +40,8 @@ adeco_status bad_number(const std::string& s) {
 }
 
 adeco_status set_string(char* dst, size_t cap, const std::string& v, const char* what) {
-    if (v.size() >= cap) return fail(ADECO_EOVERFLOW, std::string("parse_parameter_file(): ") + what + " is too long");
+    if (v.size() >= cap)
+        return fail(ADECO_EOVERFLOW, std::string(what) + " is too long (at most " + std::to_string(cap - 1) + " bytes)");
     std::memcpy(dst, v.data(), v.size());
     dst[v.size()] = 0;
     return ADECO_OK;
