@@ -29,6 +29,7 @@ FLAGS = [
     "-Wall",
     "-Wno-unused-command-line-argument",
 ]
+LIBS = ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx"]  # roctx ranges for rocprofv3 --marker-trace
 
 
 def sources():
@@ -53,7 +54,7 @@ def build_cli(verbose: bool = False) -> str:
 def build(verbose: bool = False) -> str:
     srcs = sources()
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, *srcs, "-o", tmp]
+    cmd = [HIPCC, *FLAGS, *srcs, *LIBS, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

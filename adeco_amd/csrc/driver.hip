@@ -202,6 +202,7 @@ adeco_status adeco_deposit_workflow(const adeco_parameters* p, const adeco_workf
     auto t0 = std::chrono::steady_clock::now();
     adeco_lattice L = nullptr;
     adeco_status st = ADECO_OK;
+    TraceRange phase("adeco:read_particles");
     if (sorted) {
         int64_t n = 0;
         int d[3];
@@ -220,6 +221,7 @@ adeco_status adeco_deposit_workflow(const adeco_parameters* p, const adeco_workf
     for (int a = 0; a < 3; ++a) r.particle_dims[a] = L->np[a];
     r.num_particles = L->n_vertices;
     r.load_s = seconds_since(t0);
+    phase.next("adeco:bbox_periodic_octree");
     // global bbox, periodic correction, region of interest (AdDepositWorkFlow.h:101-115)
     t0 = std::chrono::steady_clock::now();
     if ((st = adeco_lattice_bbox(L, r.global_bbox))) return st;
@@ -236,12 +238,14 @@ adeco_status adeco_deposit_workflow(const adeco_parameters* p, const adeco_workf
     r.num_leaves = adeco_octree_num_leaves(T);
     r.octree_s = seconds_since(t0);
     // leaf deposit and leaf files (deposit_leaf_nodes, AdDynamicLoadBalancing.h:983-1323)
+    phase.next("adeco:leaf_deposit");
     t0 = std::chrono::steady_clock::now();
     if ((st = adeco_deposit_leaves(T, p->linear_patch_resolution, opt->decomposition, opt->tet_mass, opt->mode,
                                    opt->precision)))
         return st;
     r.kernel_ms = adeco_last_kernel_ms();
     r.deposit_s = seconds_since(t0);
+    phase.next("adeco:leaf_files");
     t0 = std::chrono::steady_clock::now();
     if ((st = adeco_write_leaves(T, p->output_directory, nullptr, &r.total_mass))) return st;
     r.write_s = seconds_since(t0);

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstddef>
 #include <mutex>
@@ -49,6 +50,20 @@ struct DeviceState {
     size_t total_mem = 0;
     DevBuf pos, grid, stats, scratch;
     adeco_status ensure(int dev);
+};
+
+// A named roctx range over a host scope (rocprofv3 --marker-trace shows it); next() ends it and
+// starts the following phase.  It measures host time, so it covers a phase's GPU work where the
+// phase ends with a synchronisation.
+struct TraceRange {
+    explicit TraceRange(const char* name) { roctxRangePushA(name); }
+    void next(const char* name) {
+        roctxRangePop();
+        roctxRangePushA(name);
+    }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
 };
 
 DeviceState& device_state(int dev);

@@ -344,7 +344,9 @@ adeco_status adeco_deposit_leaves(adeco_octree T, int dims, int decomposition, d
     hipError_t e = hipSetDevice(T->L->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     adeco_status st;
+    TraceRange phase("adeco:bin_tets");
     if (T->binned_decomposition != decomposition && (st = octree_bin_tets(T, decomposition))) return st;
+    phase.next("adeco:deposit_leaves");
     const int64_t nl = int64_t(T->leaves_dfs.size());
     const size_t bytes = size_t(nl) * size_t(dims) * dims * dims * sizeof(double);
     const size_t budget = memory_budget(ds);
