@@ -399,7 +399,7 @@ inline void deposit_tet(const T v[4][3], G tet_mass, const int dims[3], const Bo
     }
 }
 
-// ---------------------------------------------------------------- meta-patch driver
+// ---------------------------------------------------------------- meta-patch loop
 // AdDepositTets2Grid.h:917-1053 and AdDeposit2Grid.cpp:67-125
 template <class G, class T>
 inline void deposit_patch_cell(int cell, const float* patch, int m, const Box<T>& gb, int dims,
