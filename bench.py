@@ -396,6 +396,9 @@ def program_study(n=N, runs=3, with_reference=True):
         if with_reference:
             p.input_file, p.id_type = path, "SORTED_BY_ID"
             out["cpu_baseline"] = reference_program_time(path, p, n, tmp)
+            if out["cpu_baseline"].get("value"):
+                out["vs_reference_files"] = compare_leaf_files(os.path.join(tmp, f"out_runs{runs}"),
+                                                               os.path.join(tmp, "ref_out"), DIMS)
     out["dark_sky_median_wall_s"] = float(np.median([r["wall_s"] for r in out["dark_sky_runs"]]))
     w = float(np.median([r["wall_s"] for r in out["runs"]]))
     out["median_wall_s"] = w
@@ -403,6 +406,30 @@ def program_study(n=N, runs=3, with_reference=True):
     if out.get("cpu_baseline", {}).get("value"):
         out["cpu_baseline"]["gpu_speedup"] = out["cpu_baseline"]["wall_s"] / w
     return out
+
+
+def compare_leaf_files(ours_dir, ref_dir, dims):
+    """Checker leg of the program study: this program's leaf files against the reference
+    program's on the same snapshot - the same names, and per file the largest relative density
+    difference (floor: 1e-3 of the file's largest density) and the mass difference.  The
+    reference accumulates in float, so its own error sets these figures."""
+    ours, ref = sorted(os.listdir(ours_dir)), sorted(os.listdir(ref_dir))
+    worst, worst_mass = 0.0, 0.0
+    total_ours = total_ref = 0.0
+    for f in ours:
+        if f not in ref:
+            continue
+        a = np.fromfile(os.path.join(ours_dir, f), np.float32).astype(np.float64)
+        b = np.fromfile(os.path.join(ref_dir, f), np.float32).astype(np.float64)
+        floor = 1e-3 * max(float(np.abs(b).max()), 1e-30)
+        worst = max(worst, float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))))
+        sa, sb = float(a.sum()), float(b.sum())
+        total_ours, total_ref = total_ours + sa, total_ref + sb
+        worst_mass = max(worst_mass, abs(sa - sb) / max(abs(sb), 1e-30))
+    res = {"same_file_names": ours == ref, "files": len(ours), "max_rel_density_diff": worst,
+           "max_rel_leaf_mass_diff": worst_mass, "total_rel_diff": abs(total_ours - total_ref) / max(total_ref, 1e-30)}
+    log(f"[program] vs the reference program's files: {res}")
+    return res
 
 
 def reference_program_time(snapshot_path, params, params_n, tmp):
