@@ -397,8 +397,19 @@ def program_study(n=N, runs=3, with_reference=True):
             p.input_file, p.id_type = path, "SORTED_BY_ID"
             out["cpu_baseline"] = reference_program_time(path, p, n, tmp)
             if out["cpu_baseline"].get("value"):
-                out["vs_reference_files"] = compare_leaf_files(os.path.join(tmp, f"out_runs{runs}"),
-                                                               os.path.join(tmp, "ref_out"), DIMS)
+                ref_dir = os.path.join(tmp, "ref_out")
+                out["vs_reference_files"] = compare_leaf_files(os.path.join(tmp, f"out_runs{runs}"), ref_dir, DIMS)
+                # the same in the reference's geometry (float leaf boxes, 1/dx scaling), which
+                # separates the geometry's share of the differences from the float error's
+                p.output_directory = os.path.join(tmp, "out_reference_geometry")
+                os.mkdir(p.output_directory)
+                saved = A.get_cell_scaling()
+                A.set_cell_scaling(A.SCALING_REFERENCE)
+                try:
+                    deposit_workflow(p)
+                finally:
+                    A.set_cell_scaling(saved)
+                out["vs_reference_files_reference_geometry"] = compare_leaf_files(p.output_directory, ref_dir, DIMS)
     out["dark_sky_median_wall_s"] = float(np.median([r["wall_s"] for r in out["dark_sky_runs"]]))
     w = float(np.median([r["wall_s"] for r in out["runs"]]))
     out["median_wall_s"] = w
