@@ -63,6 +63,14 @@ PROGRAM_CASES = {
                    "stride": "1", "max_tets_per_octree_node": "4000", "linear_patch_resolution": "8",
                    "num_mpi_ranks_per_node": "1", "num_pthreads": "1", "periodic_boundaries": "true"},
     },
+    # every second particle per axis of a 26^3 DARK_SKY snapshot (13^3 kept)
+    "dark_sky_stride2": {
+        "snapshot": (26, 1.5, 8),
+        "format": "DARK_SKY",
+        "params": {"file_format": "DARK_SKY", "use_region_of_interest": "false", "max_refinement_level": "3",
+                   "stride": "2", "max_tets_per_octree_node": "1500", "linear_patch_resolution": "8",
+                   "num_mpi_ranks_per_node": "1", "num_pthreads": "1", "periodic_boundaries": "true"},
+    },
     "roi_not_periodic": {
         "snapshot": (20, 1.5, 5),
         "params": {"file_format": "SORTED_BY_ID", "use_region_of_interest": "true",
@@ -71,6 +79,19 @@ PROGRAM_CASES = {
                    "num_mpi_ranks_per_node": "1", "num_pthreads": "1", "periodic_boundaries": "false"},
     },
 }
+
+
+def case_lattice(case: dict):
+    """The lattice a case's program run deposits: the snapshot, every stride-th particle per axis
+    for a DARK_SKY file read with a stride (the SORTED_BY_ID reader ignores the stride)."""
+    from adeco_amd.lattice import Lattice, make_snapshot
+    lat = make_snapshot(*case["snapshot"][:2], seed=case["snapshot"][2])
+    s = int(case["params"].get("stride", "1"))
+    if case.get("format") != "DARK_SKY" or s == 1:
+        return lat
+    n = lat.dims[0]
+    sub = [c.reshape(n, n, n)[::s, ::s, ::s].ravel().copy() for c in (lat.x, lat.y, lat.z)]
+    return Lattice(n=n // s - 1, box=lat.box, x=sub[0], y=sub[1], z=sub[2])
 
 
 def write_case_snapshot(case: dict, path: str, lat) -> None:

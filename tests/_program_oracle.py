@@ -9,11 +9,11 @@ from __future__ import annotations
 
 import numpy as np
 
-from adeco_amd.lattice import Lattice, make_snapshot, meta_patches
+from adeco_amd.lattice import Lattice, meta_patches
 from tests import _oracle as O
 from tests._golden_leaves import patch_density, patch_file_name
 from tests._octree_io import oracle_octree
-from tests._program_cases import PROGRAM_CASES
+from tests._program_cases import PROGRAM_CASES, case_lattice
 
 
 def unwrapped(x, y, z, dims, bbox, period=None):
@@ -31,7 +31,7 @@ def oracle_program(case_name, fp64=False):
     the corrected lattice, the octree).  fp64: the FP64 deposit in the reference's geometry (float
     leaf boxes, 1/dx scaling) and FP64 densities instead."""
     case = PROGRAM_CASES[case_name]
-    lat = make_snapshot(*case["snapshot"][:2], seed=case["snapshot"][2])
+    lat = case_lattice(case)
     p = case["params"]
     gb = lat.bbox()
     if p["periodic_boundaries"] == "true":

@@ -9,8 +9,8 @@ ref_program.json next to this script:
   dark_sky  AdDarkSkyDataReader's header sizes and errors, and its loads of a shuffled DARK_SKY file
             at several strides (the positions in id order: the one-rank redistribution's lattice)
   program   the reference program (main.cpp:187-249) run on SORTED_BY_ID and DARK_SKY snapshots: its leaf file
-            names, their sha256, the total mass it prints and the sha256 of the corrected
-            coordinates (apply_boundary_corrections with the snapshot's bbox)
+            names, their sha256, the total mass it prints, and the sha256 of the deposited lattice
+            and of its corrected coordinates (apply_boundary_corrections with the lattice's bbox)
 
 Each program run happens in a child process: the program finalises MPI at its end.
 """
@@ -34,8 +34,8 @@ sys.path.insert(0, ROOT)
 
 from adeco_amd.lattice import make_snapshot, write_dark_sky, write_sorted_by_id  # noqa: E402
 from tests import _oracle as O  # noqa: E402
-from tests._program_cases import (DARK_SKY_HEADERS, PARAM_TEXTS, PROGRAM_CASES, param_file_text,  # noqa: E402
-                                  unwrap_lattice_case, write_case_snapshot)
+from tests._program_cases import (DARK_SKY_HEADERS, PARAM_TEXTS, PROGRAM_CASES, case_lattice,  # noqa: E402
+                                  param_file_text, unwrap_lattice_case, write_case_snapshot)
 
 
 def sha(a) -> str:
@@ -148,6 +148,7 @@ def program_fixture(r, tmp):
         m = re.search(r"total mass\s*=\s*([0-9.eE+-]+)", run.stdout)
         files = {f: hashlib.sha256(open(os.path.join(d, "out", f), "rb").read()).hexdigest()
                  for f in sorted(os.listdir(os.path.join(d, "out")))}
+        lat = case_lattice(case)  # what the program deposits (a DARK_SKY stride subsamples)
         bbox = lat.bbox()
         x, y, z = lat.x.copy(), lat.y.copy(), lat.z.copy()
         unwrap_sha = None

@@ -21,8 +21,8 @@ import adeco_amd as A
 from adeco_amd.lattice import make_snapshot, morton_ids, write_sorted_by_id
 from adeco_amd.workflow import AMD_Parameters, get_reader, parse_parameter_file
 from tests import _oracle as O
-from tests._program_cases import (DARK_SKY_HEADERS, PARAM_TEXTS, PROGRAM_CASES, param_file_text,
-                                  unwrap_lattice_case)
+from tests._program_cases import (DARK_SKY_HEADERS, PARAM_TEXTS, PROGRAM_CASES, case_lattice,
+                                  param_file_text, unwrap_lattice_case)
 from tests._program_oracle import oracle_program, unwrapped
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -198,7 +198,7 @@ def test_oracle_periodic_correction_closed_form():
 def test_oracle_chain_reproduces_reference_program(tmp_path, name):
     g = GOLDEN["program"][name]
     case = PROGRAM_CASES[name]
-    lat = make_snapshot(*case["snapshot"][:2], seed=case["snapshot"][2])
+    lat = case_lattice(case)  # the deposited lattice (a DARK_SKY stride keeps every s-th particle)
     assert sha(np.concatenate([lat.x, lat.y, lat.z])) == g["snapshot_sha"]
     assert lat.bbox().tolist() == g["global_bbox"]
     files, total, ulat, _ = oracle_program(name)
