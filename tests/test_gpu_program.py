@@ -198,3 +198,21 @@ def test_program_forms_and_exact_geometry(tmp_path):
     assert res["num_leaves"] == len(files)
     py = {f: np.fromfile(os.path.join(p.output_directory, f), np.float32) for f in os.listdir(p.output_directory)}
     same(py, res["total_mass"], "python")
+
+
+def test_write_leaves_errors(tmp_path):
+    """Writing leaves before any deposit is EINVAL; an unwritable directory is EIO (the reference
+    throws AdRuntimeException); both leave the tree usable."""
+    lat = make_snapshot(10, 1.0, seed=2)
+    print("[program] write_leaves error paths", flush=True)
+    with A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 1000, 2) as tree:
+        with pytest.raises(A.AdRuntimeException) as e:
+            tree.write_leaves(str(tmp_path))
+        assert e.value.status == A.ADECO_EINVAL
+        tree.deposit(4, decomposition=A.DECOMP_REF_EFFECTIVE, tet_mass=1.0, mode=A.MODE_BATCHED)
+        with pytest.raises(A.AdRuntimeException) as e:
+            tree.write_leaves(str(tmp_path / "missing" / "dir"))
+        assert e.value.status == A.ADECO_EIO
+        masses, total = tree.write_leaves(str(tmp_path))
+        assert len(os.listdir(tmp_path)) == tree.num_leaves
+        assert abs(total - tree.leaf_grids().sum()) <= 1e-6 * total
