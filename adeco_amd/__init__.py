@@ -81,6 +81,25 @@ def get_cell_scaling() -> int:
     return load().adeco_get_cell_scaling()
 
 
+def set_boundary_form(decomposition=None, tet_mass=None, mode=None, precision=None):
+    """Library-wide form of DepositWrapper.deposit_tets (the boundary call): the tets of a cube
+    (default DECOMP_REF_EFFECTIVE), the mass of one tet (default 1), the kernel form (default
+    MODE_PLAIN; MODE_BATCHED runs the batched persistent deposit on the patches' tets) and the
+    precision (default PREC_FP64; the other modes need MODE_BATCHED).  Returns the previous form."""
+    lib = load()
+    prev = {"decomposition": lib.adeco_get_tet_decomposition(), "tet_mass": lib.adeco_get_tet_mass(),
+            "mode": lib.adeco_get_deposit_mode(), "precision": lib.adeco_get_precision_mode()}
+    if decomposition is not None:
+        _check(lib.adeco_set_tet_decomposition(int(decomposition)), "set_tet_decomposition()")
+    if tet_mass is not None:
+        _check(lib.adeco_set_tet_mass(float(tet_mass)), "set_tet_mass()")
+    if mode is not None:
+        _check(lib.adeco_set_deposit_mode(int(mode)), "set_deposit_mode()")
+    if precision is not None:
+        _check(lib.adeco_set_precision_mode(int(precision)), "set_precision_mode()")
+    return prev
+
+
 def set_memory_budget(nbytes: int):
     """Per-call device memory budget (0 = 75% of device memory); lower values force multi-pass."""
     _check(load().adeco_set_memory_budget(int(nbytes)), "set_memory_budget()")

@@ -25,6 +25,15 @@ SIGNATURES = {
     "adeco_version": (ctypes.c_int, []),
     "adeco_device_count": (ctypes.c_int, []),
     "adeco_set_cell_scaling": (ctypes.c_int, [ctypes.c_int]),
+    "adeco_set_tet_decomposition": (ctypes.c_int, [ctypes.c_int]),
+    "adeco_get_tet_decomposition": (ctypes.c_int, []),
+    "adeco_set_tet_mass": (ctypes.c_int, [ctypes.c_double]),
+    "adeco_get_tet_mass": (ctypes.c_double, []),
+    "adeco_set_deposit_mode": (ctypes.c_int, [ctypes.c_int]),
+    "adeco_get_deposit_mode": (ctypes.c_int, []),
+    "adeco_set_precision_mode": (ctypes.c_int, [ctypes.c_int]),
+    "adeco_get_precision_mode": (ctypes.c_int, []),
+    "adeco_deposit_leaves_batched": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
     "adeco_get_cell_scaling": (ctypes.c_int, []),
     "adeco_deposit_patches": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
                                              c_vp, ctypes.c_size_t, ctypes.c_int, c_vp, ctypes.c_int, c_vp]),

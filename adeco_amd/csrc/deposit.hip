@@ -26,7 +26,7 @@ __device__ __forceinline__ void flush_counters(const TetCounters& c, unsigned lo
 
 // AoS meta-patches (3*(m+1)^3 floats each), one thread per (cell, tet) of num_patches*m^3 cells.
 // Cells with a 666 sentinel vertex are skipped whole (AdDepositTets2Grid.h:1029-1038); the 6 tets
-// use the reference's slot order, i.e. the REF_EFFECTIVE decomposition.
+// follow the corner table pc (the reference's slot order is the REF_EFFECTIVE decomposition).
 template <bool kCount>
 __global__ void __launch_bounds__(256) k_deposit_patches_plain(const float* __restrict__ pos,
                                                                int64_t n_tets, int m, GridGeom g,
@@ -112,16 +112,51 @@ __global__ void k_fill_f64(double* d, int64_t n, double v) {
         d[i] = v;
 }
 
+// One thread per tet of the meta-patches: its 4 vertices, in the corner table's order, to the
+// soup (float [n][4][3]); a cell with a sentinel vertex gives 6 degenerate tets at the origin.
+__global__ void __launch_bounds__(256) k_patches_to_soup(const float* __restrict__ pos, int64_t n_tets, int m,
+                                                         PackedCorners pc, float* __restrict__ soup) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n_tets) return;
+    const int64_t cell = t / 6;
+    const int lt = int(t - cell * 6);
+    const int cpp = m * m * m;
+    const int64_t p = cell / cpp;
+    const int c = int(cell - p * cpp);
+    const int ck = c / (m * m), rest = c - ck * m * m, cj = rest / m, ci = rest - cj * m;
+    const int64_t fpp = 3ll * (m + 1) * (m + 1) * (m + 1);
+    const float* base = pos + p * fpp + 3 * (ci + (m + 1) * (cj + (m + 1) * ck));
+    const int ax = 3, ay = 3 * (m + 1), az = 3 * (m + 1) * (m + 1);
+    const int slot_off[8] = {0, ax, ay, ay + ax, ay + az, ay + az + ax, az, az + ax};
+    bool skip = false;
+    for (int s = 0; s < 8; ++s) skip = skip || is_sentinel(base[slot_off[s]], base[slot_off[s] + 1], base[slot_off[s] + 2]);
+    float* out = soup + 12 * t;
+    for (int q = 0; q < 4; ++q) {
+        const int code = pc.code(lt, q);
+        const float* pv = base + 3 * (code & 1) + ay * ((code >> 1) & 1) + az * ((code >> 2) & 1);
+        for (int a = 0; a < 3; ++a) out[3 * q + a] = skip ? 0.f : pv[a];
+    }
+}
+
 inline unsigned blocks_for(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
 
 }  // namespace
 
-hipError_t launch_deposit_patches_plain(const float* d_positions, int64_t num_patches, int m,
+hipError_t launch_patches_to_soup(const float* d_positions, int64_t num_patches, int m, int decomp, float* d_soup,
+                                  hipStream_t stream) {
+    const int64_t n = num_patches * int64_t(m) * m * m * 6;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_patches_to_soup, dim3(blocks_for(n, 256)), dim3(256), 0, stream, d_positions, n, m,
+                       pack_corners(corner_table(decomp)), d_soup);
+    return hipGetLastError();
+}
+
+hipError_t launch_deposit_patches_plain(const float* d_positions, int64_t num_patches, int m, int decomp,
                                         const GridGeom& g, double tet_mass, double* d_grid,
                                         unsigned long long* d_stats, hipStream_t stream) {
     const int64_t n = num_patches * int64_t(m) * m * m * 6;
     if (n <= 0) return hipSuccess;
-    const PackedCorners pc = pack_corners(corner_table(REF_EFFECTIVE));
+    const PackedCorners pc = pack_corners(corner_table(decomp));
     if (d_stats)
         hipLaunchKernelGGL(k_deposit_patches_plain<true>, dim3(blocks_for(n, 256)), dim3(256), 0, stream,
                            d_positions, n, m, g, tet_mass, d_grid, d_stats, pc);

@@ -66,9 +66,13 @@ struct LatticeView {
 constexpr int kNumStats = 11;
 
 // ---- deposit.hip launchers (all asynchronous on `stream`) ----
-hipError_t launch_deposit_patches_plain(const float* d_positions, int64_t num_patches, int m,
+hipError_t launch_deposit_patches_plain(const float* d_positions, int64_t num_patches, int m, int decomp,
                                         const GridGeom& g, double tet_mass, double* d_grid,
                                         unsigned long long* d_stats, hipStream_t stream);
+// The 6 m^3 tets of every meta-patch as a soup (float [n][4][3]); cells with a sentinel vertex
+// give degenerate tets (all four vertices at the origin), which deposit nothing.
+hipError_t launch_patches_to_soup(const float* d_positions, int64_t num_patches, int m, int decomp, float* d_soup,
+                                  hipStream_t stream);
 
 hipError_t launch_deposit_lattice_plain(const LatticeView& lat, int decomp, const int32_t* d_tet_ids,
                                         int64_t n_tets, const GridGeom& g, double tet_mass,

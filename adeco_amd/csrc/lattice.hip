@@ -89,7 +89,7 @@ adeco_status deposit_soup_batched(DeviceState& ds, const float* d_tets, int64_t 
     if (n_tets == 0) return ADECO_OK;
     if (n_tets >= (int64_t(1) << 31) - 2) return fail(ADECO_EOVERFLOW, "deposit_soup(): too many tets for the batched form");
     if (g.dims > 1023) return fail(ADECO_EINVAL, "deposit_soup(): the batched form needs dims < 1024");
-    DevBuf ids, desc, work, tiles, ctr;
+    DevBuf &ids = ds.soup_ids, &desc = ds.soup_desc, &work = ds.soup_work, &tiles = ds.soup_tiles, &ctr = ds.soup_ctr;
     adeco_status st;
     if ((st = ids.reserve(size_t(n_tets) * 4)) || (st = desc.reserve(512)) ||
         (st = work.reserve(candidate_space_bytes(n_tets) + 256)) || (st = ctr.reserve(64)))
@@ -119,7 +119,7 @@ adeco_status deposit_soup_batched(DeviceState& ds, const float* d_tets, int64_t 
     if (e == hipSuccess) e = hipEventRecord(ds.evk1, ds.stream);
     unsigned long long cnt[2] = {0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(cnt, counters, 16, hipMemcpyDeviceToHost, ds.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);  // the local buffers are freed on return
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
     float ms = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ds.evk0, ds.evk1);
     if (e != hipSuccess) return hip_fail(e, "soup batched deposit");

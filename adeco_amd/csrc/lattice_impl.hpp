@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "kernels.hpp"
 #include "runtime.hpp"
 
 struct adeco_lattice_s {
@@ -36,4 +37,8 @@ namespace adeco {
 constexpr int64_t kMaxLatticeVertices = (int64_t(1) << 31) / 6;
 adeco_status lattice_bbox_device(adeco_lattice_s* L, float out6[6]);
 adeco_status lattice_alloc(int device_id, int npx, int npy, int npz, adeco_lattice_s*& out);
+// The batched persistent deposit of a tet soup (float [n][4][3], on the device) onto one grid,
+// accumulating; kernel time, partial cells and FP64 recomputes go to the thread's timings.
+adeco_status deposit_soup_batched(DeviceState& ds, const float* d_tets, int64_t n_tets, const GridGeom& g,
+                                  double tet_mass, int precision, double* d_grid);
 }

@@ -16,8 +16,10 @@
 #include <vector>
 
 #include "../../include/adeco.h"
+#include "batched_impl.hpp"
 #include "deposit_tet.hpp"
 #include "kernels.hpp"
+#include "lattice_impl.hpp"
 #include "runtime.hpp"
 
 namespace adeco {
@@ -28,6 +30,11 @@ thread_local LastTimings g_timings;
 static size_t g_memory_budget = 0;  // 0 -> 75% of device memory
 static std::mutex g_budget_mutex;
 static std::atomic<int> g_cell_scaling{kScalingExact};
+// the boundary call's form (adeco_deposit_patches): tets, tet mass, kernel form and precision
+static std::atomic<int> g_boundary_decomp{REF_EFFECTIVE};
+static std::atomic<double> g_boundary_tet_mass{1.0};  // resample_meta_patch, AdDepositTets2Grid.h:1007-1053
+static std::atomic<int> g_boundary_mode{ADECO_MODE_PLAIN};
+static std::atomic<int> g_boundary_precision{ADECO_PREC_FP64};
 
 int cell_scaling() { return g_cell_scaling.load(); }
 
@@ -135,7 +142,11 @@ using namespace adeco;
 
 namespace {
 
-// One device's share of a boundary call: passes over its patch range within the budget.
+// One device's share of a boundary call: passes over its patch range within the budget.  The
+// plain form runs one thread per tet over the uploaded patches; the batched form expands each
+// pass's patches into a tet soup on the device and runs the batched persistent deposit on it, so
+// a pass budgets the soup and the soup's candidate space as well (the tile map, about one 64th of
+// the candidates in 4 bytes each, is allocated on top).
 adeco_status deposit_patches_on_device(int dev, size_t first_patch, size_t n_patches, const float* positions,
                                        int m, const float* bbox, int dims, std::vector<double>& host_sum) {
     adeco_status st = check_device(dev);
@@ -152,12 +163,25 @@ adeco_status deposit_patches_on_device(int dev, size_t first_patch, size_t n_pat
     if (grid_bytes >= budget)  // AdDeposit2Grid.cu:314-317
         return fail(ADECO_EOVERFLOW, "grid of " + std::to_string(grid_bytes) + " bytes exceeds the device memory budget");
     const size_t avail = budget - grid_bytes;
+    const int decomp = g_boundary_decomp.load();
+    const double tet_mass = g_boundary_tet_mass.load();
+    const bool batched = g_boundary_mode.load() == ADECO_MODE_BATCHED;
+    const int precision = g_boundary_precision.load();
+    if (!batched && precision != ADECO_PREC_FP64)
+        return fail(ADECO_EINVAL, "deposit_tets(): precision modes other than FP64 need the batched form");
+    if (batched && dims > 1023) return fail(ADECO_EINVAL, "deposit_tets(): the batched form needs dims < 1024");
     const size_t patch_bytes = fpp * sizeof(float);
-    const size_t patches_per_pass = std::max<size_t>(1, std::min(n_patches, avail / patch_bytes));
-    if (patches_per_pass * patch_bytes > avail)
+    const size_t tets_per_patch = 6 * size_t(m) * m * m;
+    const size_t soup_bytes = tets_per_patch * 12 * sizeof(float);
+    const size_t per_patch = batched ? patch_bytes + soup_bytes + tets_per_patch * 4 + candidate_space_bytes(int64_t(tets_per_patch))
+                                     : patch_bytes;
+    const size_t patches_per_pass = std::max<size_t>(1, std::min(n_patches, avail / per_patch));
+    if (patches_per_pass * per_patch > avail)
         return fail(ADECO_EOVERFLOW, "memory budget too small for one meta-patch");
     if ((st = ds.grid.reserve(grid_bytes))) return st;
     if ((st = ds.pos.reserve(patches_per_pass * patch_bytes))) return st;
+    DevBuf& soup = ds.soup;
+    if (batched && (st = soup.reserve(patches_per_pass * soup_bytes))) return st;
     const GridGeom g = make_grid_geom(bbox, dims, cell_scaling());
     if ((e = launch_fill_f64(ds.grid.as<double>(), int64_t(ncell), 0.0, ds.stream)) != hipSuccess)
         return hip_fail(e, "grid clear");
@@ -167,8 +191,17 @@ adeco_status deposit_patches_on_device(int dev, size_t first_patch, size_t n_pat
         e = hipMemcpyAsync(ds.pos.ptr, positions + (first_patch + done) * fpp, np * patch_bytes,
                            hipMemcpyHostToDevice, ds.stream);
         if (e != hipSuccess) return hip_fail(e, "positions upload");
+        if (batched) {
+            e = launch_patches_to_soup(ds.pos.as<float>(), int64_t(np), m, decomp, soup.as<float>(), ds.stream);
+            if (e != hipSuccess) return hip_fail(e, "patch expansion");
+            if ((st = deposit_soup_batched(ds, soup.as<float>(), int64_t(np * tets_per_patch), g, tet_mass, precision,
+                                           ds.grid.as<double>())))
+                return st;
+            total_ms += g_timings.main_ms;
+            continue;
+        }
         if ((st = timed_begin(ds))) return st;
-        e = launch_deposit_patches_plain(ds.pos.as<float>(), int64_t(np), m, g, 1.0, ds.grid.as<double>(),
+        e = launch_deposit_patches_plain(ds.pos.as<float>(), int64_t(np), m, decomp, g, tet_mass, ds.grid.as<double>(),
                                          nullptr, ds.stream);
         if (e != hipSuccess) return hip_fail(e, "deposit launch");
         if ((st = timed_end(ds))) return st;
@@ -215,6 +248,34 @@ adeco_status adeco_set_cell_scaling(int mode) {
 }
 
 int adeco_get_cell_scaling(void) { return g_cell_scaling.load(); }
+
+adeco_status adeco_set_tet_decomposition(int decomposition) {
+    if (decomposition < 0 || decomposition > 2) return fail(ADECO_EINVAL, "set_tet_decomposition(): unknown decomposition");
+    g_boundary_decomp.store(decomposition);
+    return ADECO_OK;
+}
+int adeco_get_tet_decomposition(void) { return g_boundary_decomp.load(); }
+
+adeco_status adeco_set_tet_mass(double tet_mass) {
+    if (!(tet_mass > 0.0) || !std::isfinite(tet_mass)) return fail(ADECO_EINVAL, "set_tet_mass(): tet mass must be positive and finite");
+    g_boundary_tet_mass.store(tet_mass);
+    return ADECO_OK;
+}
+double adeco_get_tet_mass(void) { return g_boundary_tet_mass.load(); }
+
+adeco_status adeco_set_deposit_mode(int mode) {
+    if (mode != ADECO_MODE_PLAIN && mode != ADECO_MODE_BATCHED) return fail(ADECO_EINVAL, "set_deposit_mode(): unknown mode");
+    g_boundary_mode.store(mode);
+    return ADECO_OK;
+}
+int adeco_get_deposit_mode(void) { return g_boundary_mode.load(); }
+
+adeco_status adeco_set_precision_mode(int precision) {
+    if (precision < ADECO_PREC_FP64 || precision > ADECO_PREC_FP32) return fail(ADECO_EINVAL, "set_precision_mode(): unknown precision");
+    g_boundary_precision.store(precision);
+    return ADECO_OK;
+}
+int adeco_get_precision_mode(void) { return g_boundary_precision.load(); }
 
 void adeco_last_timings(double* out5) {
     if (!out5) return;

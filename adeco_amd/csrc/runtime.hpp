@@ -49,6 +49,10 @@ struct DeviceState {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
     size_t total_mem = 0;
     DevBuf pos, grid, stats, scratch;
+    // the batched deposit of a tet soup (boundary calls in the batched form, adeco_deposit_soup):
+    // the soup, tet ids, grid descriptor, candidate space, tile map and counters, reused by every
+    // call on this device (calls hold the mutex)
+    DevBuf soup, soup_ids, soup_desc, soup_work, soup_tiles, soup_ctr;
     adeco_status ensure(int dev);
 };
 

@@ -380,6 +380,10 @@ adeco_status adeco_deposit_leaves(adeco_octree T, int dims, int decomposition, d
     return ADECO_OK;
 }
 
+adeco_status adeco_deposit_leaves_batched(adeco_octree T, int dims, int decomposition, double tet_mass, int precision) {
+    return adeco_deposit_leaves(T, dims, decomposition, tet_mass, ADECO_MODE_BATCHED, precision);
+}
+
 adeco_status adeco_set_launch_shape(int chunk, int blocks_per_cu) {
     if (chunk < 1 || chunk > 1024 || blocks_per_cu < 1 || blocks_per_cu > 16)
         return fail(ADECO_EINVAL, "set_launch_shape(): chunk must be in 1..1024 and blocks_per_cu in 1..16");

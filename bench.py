@@ -483,6 +483,53 @@ def reference_program_time(snapshot_path, params, params_n, tmp):
             "total_mass": total, "kind": "reference", "sample": "the whole program on the same snapshot"}
 
 
+def boundary_study(reps=2):
+    """The drop-in boundary (DepositWrapper::deposit_tets, adeco_deposit_patches) on the 128^3
+    workload, called as the reference's worker calls it (AdDynamicLoadBalancing.h:200-205): once
+    per octree leaf, with the leaf's gathered meta-patches in host memory (its meta-cell list, the
+    reference's gather order), the leaf's float box, 16^3 cells, the reference's tets of mass 1,
+    into a zeroed float grid.  The plain form (one thread per tet, the reference kernel's
+    structure) against the batched form; host-to-device copies of the patches and the grid
+    readback are inside the timed loop."""
+    from adeco_amd.lattice import meta_patches
+    lat = make_lattice(N, RMS, seed=SEED)
+    with A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, MAX_TETS, MAX_LEVEL) as tree:
+        lv = tree.leaves()
+        boxes = tree.leaf_boxes()
+    t0 = time.perf_counter()
+    patches = [meta_patches(lat, lv["mc_ids"][lv["mc_offsets"][l]:lv["mc_offsets"][l + 1]]) for l in range(boxes.shape[0])]
+    log(f"[boundary] gathered {sum(p.size for p in patches) * 4 / 1e6:.0f} MB of meta-patches for "
+        f"{len(patches)} leaves in {time.perf_counter() - t0:.1f}s")
+    out = {"leaves": len(patches), "patch_bytes": int(sum(p.size for p in patches) * 4), "forms": {}}
+    grids = {}
+    for name, mode in (("plain", A.MODE_PLAIN), ("batched", A.MODE_BATCHED)):
+        prev = A.set_boundary_form(mode=mode)
+        try:
+            best = None
+            for _ in range(reps + 1):
+                res = []
+                t0 = time.perf_counter()
+                for p, box in zip(patches, boxes):
+                    g = np.zeros(DIMS ** 3, np.float32)
+                    if p.size:
+                        A.DepositWrapper.deposit_tets(True, 0, 0, p.size // 1536, p, p.size, 7, box, DIMS, g)
+                    res.append(g)
+                wall = time.perf_counter() - t0
+                best = wall if best is None else min(best, wall)
+            grids[name] = res
+        finally:
+            A.set_boundary_form(**prev)
+        mass = float(sum(float(g.sum(dtype=np.float64)) for g in grids[name]))
+        out["forms"][name] = {"s": best, "tets_per_s": 6 * (N ** 3) / best, "total_mass": mass}
+        log(f"[boundary] {name}: {best * 1e3:.1f} ms for {len(patches)} calls ({6 * N ** 3 / best:.3e} tets/s), "
+            f"mass {mass:.6f}")
+    err = max_rel_err(np.concatenate(grids["batched"]), np.concatenate(grids["plain"]), 1e-8)
+    out["batched_over_plain"] = out["forms"]["plain"]["s"] / out["forms"]["batched"]["s"]
+    out["batched_vs_plain_max_rel_err"] = err
+    log(f"[boundary] batched / plain: {out['batched_over_plain']:.1f}x, per-cell max rel err {err:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -494,6 +541,8 @@ def main():
     ap.add_argument("--serve", action="store_true", help="run only the ROI serving study (256^3 lattice)")
     ap.add_argument("--precision", action="store_true", help="run only the precision-mode study")
     ap.add_argument("--heavy-tail", action="store_true", help="run only the 256^3 heavy-tailed batched study")
+    ap.add_argument("--boundary", action="store_true",
+                    help="run only the drop-in boundary study (per-leaf deposit_tets calls, plain vs batched)")
     ap.add_argument("--program", action="store_true",
                     help="run only the whole single-rank program (file in, leaf files out) on a 128^3 snapshot")
     ap.add_argument("--serve-rates", default="20,50,100,200")
@@ -507,6 +556,12 @@ def main():
         print(json.dumps({"metric": "tets/s exact deposit, batched vs plain per-leaf launches", "unit": "tets/s",
                           "higher_is_better": True, "n_gpus": 1, "dtype": "f64",
                           "data": "synthetic (256^3 lattice, rms 4 spacings + 64 infall centres)", **res}))
+        return
+    if args.boundary:
+        res = boundary_study()
+        print(json.dumps({"metric": "tets/s through DepositWrapper::deposit_tets (per-leaf calls, PCIe included)",
+                          "unit": "tets/s", "value": res["forms"]["batched"]["tets_per_s"], "higher_is_better": True,
+                          "n_gpus": 1, "dtype": "f64", "data": "synthetic (128^3 lattice, rms 3 spacings)", **res}))
         return
     if args.program:
         res = program_study(with_reference=not args.no_cpu_baseline)

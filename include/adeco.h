@@ -66,6 +66,21 @@ typedef int adeco_status;
 #define ADECO_SCALING_REFERENCE 1
 
 const char* adeco_last_error(void);
+
+/* Library-wide form of the boundary call adeco_deposit_patches (set before concurrent use): the 6
+ * tets of a lattice cube (default ADECO_DECOMP_REF_EFFECTIVE, the reference's), the mass of one tet
+ * (default 1, as resample_meta_patch, AdDepositTets2Grid.h:1007-1053), the kernel form (default
+ * ADECO_MODE_PLAIN, one thread per tet like the reference's kernel; ADECO_MODE_BATCHED expands
+ * each pass's patches into a tet soup on the device and runs the batched persistent deposit on it)
+ * and the precision (default ADECO_PREC_FP64; MIXED and FP32 need the batched form). */
+adeco_status adeco_set_tet_decomposition(int decomposition);
+int adeco_get_tet_decomposition(void);
+adeco_status adeco_set_tet_mass(double tet_mass);
+double adeco_get_tet_mass(void);
+adeco_status adeco_set_deposit_mode(int mode);
+int adeco_get_deposit_mode(void);
+adeco_status adeco_set_precision_mode(int precision);
+int adeco_get_precision_mode(void);
 int adeco_version(void);
 adeco_status adeco_set_cell_scaling(int mode);
 int adeco_get_cell_scaling(void);
@@ -161,6 +176,9 @@ adeco_status adeco_octree_bin(adeco_octree tree, int decomposition, int64_t* lea
  * adeco_last_kernel_ms. */
 adeco_status adeco_deposit_leaves(adeco_octree tree, int dims, int decomposition, double tet_mass, int mode,
                                   int precision);
+/* adeco_deposit_leaves in the batched persistent form (ADECO_MODE_BATCHED). */
+adeco_status adeco_deposit_leaves_batched(adeco_octree tree, int dims, int decomposition, double tet_mass,
+                                          int precision);
 /* Copies leaf grids [first_leaf, first_leaf + n_leaves) of the last deposit (FP64 mass per cell,
  * i fastest, leaf-major in DFS leaf order). */
 adeco_status adeco_leaf_grids(adeco_octree tree, int64_t first_leaf, int64_t n_leaves, double* out);

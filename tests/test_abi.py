@@ -220,3 +220,22 @@ def test_kernel_per_tet_deposit_matches_oracle_on_host(rms, dims):
     assert got.sum() == pytest.approx(6 * lat.n ** 3, rel=1e-12)
     # the soup in lattice order equals the oracle's lattice deposit
     assert O.max_rel_err(got, O.deposit_lattice(lat, box, dims, decomp=2), floor_frac=1e-6) <= 1e-9
+
+
+def test_boundary_form_settings():
+    """The boundary call's form settings validate their arguments and keep their values (no GPU)."""
+    lib = A.load()
+    assert (lib.adeco_get_tet_decomposition(), lib.adeco_get_tet_mass(), lib.adeco_get_deposit_mode(),
+            lib.adeco_get_precision_mode()) == (A.DECOMP_REF_EFFECTIVE, 1.0, A.MODE_PLAIN, A.PREC_FP64)
+    assert lib.adeco_set_tet_decomposition(3) == A.ADECO_EINVAL
+    assert lib.adeco_set_tet_mass(0.0) == A.ADECO_EINVAL
+    assert lib.adeco_set_tet_mass(float("inf")) == A.ADECO_EINVAL
+    assert lib.adeco_set_deposit_mode(2) == A.ADECO_EINVAL
+    assert lib.adeco_set_precision_mode(3) == A.ADECO_EINVAL
+    prev = A.set_boundary_form(decomposition=A.DECOMP_KUHN, tet_mass=0.25, mode=A.MODE_BATCHED, precision=A.PREC_MIXED)
+    try:
+        assert A.set_boundary_form() == {"decomposition": A.DECOMP_KUHN, "tet_mass": 0.25, "mode": A.MODE_BATCHED,
+                                         "precision": A.PREC_MIXED}
+    finally:
+        A.set_boundary_form(**prev)
+    assert A.set_boundary_form() == prev

@@ -167,3 +167,45 @@ def test_batched_column_segments_on_fine_grids(dims):
     assert O.max_rel_err(got, want, tet_mass=1.0 / 6) <= REL
     assert O.max_rel_err(plain, want, tet_mass=1.0 / 6) <= REL
     assert abs(got.sum() - want.sum()) <= 1e-9 * want.sum()
+
+
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_MIXED])
+def test_boundary_batched_form_matches_oracle(precision):
+    """DepositWrapper.deposit_tets in the batched form (the patches expanded into a tet soup on the
+    device, then the batched persistent deposit): the oracle's grid to 1e-6, also over many passes
+    of a small memory budget, and with another decomposition and tet mass."""
+    print(f"[boundary] batched form, precision {precision}", flush=True)
+    lat = make_lattice(16, 2.0, seed=7)
+    pat = meta_patches(lat)
+    npatch = pat.size // 1536
+    bbox = A.cubic_root_box(lat.bbox())
+    dims = 24
+    want = O.deposit_patches_ld(pat, bbox, dims)
+    want_kuhn = O.deposit_lattice(lat, bbox, dims, decomp=A.DECOMP_KUHN, tet_mass=1.0 / 6.0, precision=3)
+    prev = A.set_boundary_form(mode=A.MODE_BATCHED, precision=precision)
+    try:
+        got = np.zeros(dims ** 3, np.float32)
+        A.DepositWrapper.deposit_tets(True, 0, 0, npatch, pat, pat.size, 7, bbox, dims, got)
+        assert O.max_rel_err(got, want) <= REL
+        # about two patches per pass (each holds its patch, soup and candidate space)
+        A.set_memory_budget(dims ** 3 * 8 + 1_200_000)
+        multi = np.zeros(dims ** 3, np.float32)
+        A.DepositWrapper.deposit_tets(True, 0, 0, npatch, pat, pat.size, 7, bbox, dims, multi)
+        assert O.max_rel_err(multi, want) <= REL
+        A.set_memory_budget(0)
+        A.set_boundary_form(decomposition=A.DECOMP_KUHN, tet_mass=1.0 / 6.0)
+        kuhn = np.zeros(dims ** 3, np.float32)
+        A.DepositWrapper.deposit_tets(True, 0, 0, npatch, pat, pat.size, 7, bbox, dims, kuhn)
+        assert O.max_rel_err(kuhn, want_kuhn, tet_mass=1.0 / 6.0) <= REL
+        # the plain form with the same decomposition and mass
+        A.set_boundary_form(mode=A.MODE_PLAIN, precision=A.PREC_FP64)
+        plain = np.zeros(dims ** 3, np.float32)
+        A.DepositWrapper.deposit_tets(True, 0, 0, npatch, pat, pat.size, 7, bbox, dims, plain)
+        assert O.max_rel_err(plain, want_kuhn, tet_mass=1.0 / 6.0) <= REL
+        # precision modes other than FP64 need the batched form
+        A.set_boundary_form(precision=A.PREC_MIXED)
+        with pytest.raises(A.AdRuntimeException, match="batched form"):
+            A.DepositWrapper.deposit_tets(True, 0, 0, npatch, pat, pat.size, 7, bbox, dims, plain)
+    finally:
+        A.set_memory_budget(0)
+        A.set_boundary_form(**prev)
