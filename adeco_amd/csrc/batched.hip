@@ -80,6 +80,8 @@ constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = 64 * kWavesPerBlock;
 constexpr int kBuckets = 10;       // 9 clip work classes + the exact-recompute queue
 constexpr int kExactQueue = 9;
+// launches with fewer tiles than this per resident wave drain their queues in merged passes
+constexpr int64_t kMergeDrainTiles = 16;
 constexpr int kQueueCap = 2 * kWaveTile;
 
 // Per-wavefront LDS: pair-start markers of the current tile, the bucketed partial-candidate queues
@@ -574,7 +576,7 @@ struct ManySources {
 // directly (one source) or through `entries`.  The wave pops `chunk` items per work-queue access;
 // a queue reaching 64 entries is clipped at once; when the items run out the queues are drained.
 // Every wave leaves the loop once the counter passes n_items and its queues are empty.
-template <int kPrec, class Sources>
+template <int kPrec, bool kMergeDrain, class Sources>
 __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __restrict__ entries, int64_t n_items,
                                           int chunk, unsigned* __restrict__ queue_counter,
                                           unsigned long long* __restrict__ counters) {
@@ -619,19 +621,46 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
         if (ready) {
             const int b = (ready >> kExactQueue) & 1ull ? kExactQueue : __builtin_ctzll(ready);
             const int cnt = __builtin_amdgcn_readfirstlane(__shfl(qv, b, 64));
-            const int take = min(cnt, kWaveTile);
-            const int rest = cnt - take;
+            // this pass's entry of each lane: from queue b, or when draining a small launch with no
+            // full queue, up to 64 entries across all the work classes (they only group similar
+            // work; the clip does not depend on them), from the top of each queue, lowest class
+            // first.  Either way a pass takes >= 1 entry, so the drain ends.
             int2 e = make_int2(0, 0);
             int slot = 0;
-            if (lane < take) {
-                e = L.q[b][rest + lane];
-                slot = L.qs[b][rest + lane];
+            bool have = false;
+            if (kMergeDrain && draining && b != kExactQueue && cnt < kWaveTile) {
+                const int qc = lane < kExactQueue ? qv : 0;
+                int pre = qc;
+                for (int d = 1; d < 16; d <<= 1) {
+                    const int o = __shfl_up(pre, d, 64);
+                    if (lane >= d) pre += o;
+                }
+                const int excl = pre - qc;
+                const int tq = lane < kExactQueue ? max(0, min(qc, kWaveTile - excl)) : 0;
+                for (int qi = 0; qi < kExactQueue; ++qi) {
+                    const int ex = __shfl(excl, qi, 64), tk = __shfl(tq, qi, 64), cq = __shfl(qc, qi, 64);
+                    if (lane >= ex && lane < ex + tk) {
+                        e = L.q[qi][cq - tk + (lane - ex)];
+                        slot = L.qs[qi][cq - tk + (lane - ex)];
+                        have = true;
+                    }
+                }
+                wave_sync();
+                if (lane < kExactQueue) L.qn[lane] = qc - tq;
+            } else {
+                const int take = min(cnt, kWaveTile);
+                const int rest = cnt - take;
+                if (lane < take) {
+                    e = L.q[b][rest + lane];
+                    slot = L.qs[b][rest + lane];
+                    have = true;
+                }
+                wave_sync();
+                if (lane == 0) L.qn[b] = rest;
             }
             wave_sync();
-            if (lane == 0) L.qn[b] = rest;
-            wave_sync();
             // (a pass over another queue starts only with no cells pending; exact clips leave none)
-            if (lane < take) {
+            if (have) {
                 const unsigned redo =
                     clip_entry<kPrec>(srcs, e, slot, b == kExactQueue, &L.acc[0][lane], &L.sab[0][lane]);
                 if (redo) {
@@ -665,10 +694,10 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
     if (fb) atomicAdd(counters, fb);
 }
 
-template <int kPrec>
+template <int kPrec, bool kMergeDrain>
 __global__ void __launch_bounds__(kBlock, 2) k_deposit_waves(TileSource src, int chunk, unsigned* __restrict__ queue_counter,
                                                           unsigned long long* __restrict__ counters) {
-    wave_loop<kPrec>(OneSource{src}, nullptr, src.n_tiles, chunk, queue_counter, counters);
+    wave_loop<kPrec, kMergeDrain>(OneSource{src}, nullptr, src.n_tiles, chunk, queue_counter, counters);
 }
 
 template <int kPrec>
@@ -676,7 +705,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_deposit_step(const TileSource* __
                                                          const int2* __restrict__ entries, int n_entries, int chunk,
                                                          unsigned* __restrict__ queue_counter,
                                                          unsigned long long* __restrict__ counters) {
-    wave_loop<kPrec>(ManySources{sources}, entries, n_entries, chunk, queue_counter, counters);
+    wave_loop<kPrec, false>(ManySources{sources}, entries, n_entries, chunk, queue_counter, counters);
 }
 
 __global__ void k_zero_words(unsigned* p, int n) {
@@ -750,16 +779,31 @@ adeco_status build_tile_map(hipStream_t stream, CandidateSpace& cs, int32_t* til
 hipError_t launch_deposit_tiles(hipStream_t stream, int precision, const TileSource& src, unsigned* queue_counter,
                                 unsigned long long* counters, int device) {
     if (src.n_tiles <= 0) return hipSuccess;
-    const int chunk = std::max(1, launch_shape().chunk);
+    // small launches: fewer tiles per pop when there are too few to give every resident wave
+    // `chunk` of them, and drain passes that merge the work classes
+    const int64_t resident_waves = int64_t(resident_blocks(device)) * kWavesPerBlock;
+    const int chunk = int(std::max<int64_t>(1, std::min<int64_t>(launch_shape().chunk, src.n_tiles / resident_waves)));
+    const bool merge = src.n_tiles < kMergeDrainTiles * resident_waves;
     const int64_t waves_needed = (src.n_tiles + chunk - 1) / chunk;
     const unsigned grid = unsigned(std::max<int64_t>(1, std::min<int64_t>((waves_needed + kWavesPerBlock - 1) / kWavesPerBlock,
                                                                           resident_blocks(device))));
-    if (precision == ADECO_PREC_FP64)
-        hipLaunchKernelGGL(k_deposit_waves<ADECO_PREC_FP64>, dim3(grid), dim3(kBlock), 0, stream, src, chunk, queue_counter, counters);
-    else if (precision == ADECO_PREC_MIXED)
-        hipLaunchKernelGGL(k_deposit_waves<ADECO_PREC_MIXED>, dim3(grid), dim3(kBlock), 0, stream, src, chunk, queue_counter, counters);
-    else
-        hipLaunchKernelGGL(k_deposit_waves<ADECO_PREC_FP32>, dim3(grid), dim3(kBlock), 0, stream, src, chunk, queue_counter, counters);
+    const dim3 g3(grid), b3(kBlock);
+    if (precision == ADECO_PREC_FP64) {
+        if (merge)
+            hipLaunchKernelGGL((k_deposit_waves<ADECO_PREC_FP64, true>), g3, b3, 0, stream, src, chunk, queue_counter, counters);
+        else
+            hipLaunchKernelGGL((k_deposit_waves<ADECO_PREC_FP64, false>), g3, b3, 0, stream, src, chunk, queue_counter, counters);
+    } else if (precision == ADECO_PREC_MIXED) {
+        if (merge)
+            hipLaunchKernelGGL((k_deposit_waves<ADECO_PREC_MIXED, true>), g3, b3, 0, stream, src, chunk, queue_counter, counters);
+        else
+            hipLaunchKernelGGL((k_deposit_waves<ADECO_PREC_MIXED, false>), g3, b3, 0, stream, src, chunk, queue_counter, counters);
+    } else {
+        if (merge)
+            hipLaunchKernelGGL((k_deposit_waves<ADECO_PREC_FP32, true>), g3, b3, 0, stream, src, chunk, queue_counter, counters);
+        else
+            hipLaunchKernelGGL((k_deposit_waves<ADECO_PREC_FP32, false>), g3, b3, 0, stream, src, chunk, queue_counter, counters);
+    }
     return hipGetLastError();
 }
 
