@@ -700,12 +700,12 @@ __global__ void __launch_bounds__(kBlock, 2) k_deposit_waves(TileSource src, int
     wave_loop<kPrec, kMergeDrain>(OneSource{src}, nullptr, src.n_tiles, chunk, queue_counter, counters);
 }
 
-template <int kPrec>
+template <int kPrec, bool kMergeDrain>
 __global__ void __launch_bounds__(kBlock, 2) k_deposit_step(const TileSource* __restrict__ sources,
                                                          const int2* __restrict__ entries, int n_entries, int chunk,
                                                          unsigned* __restrict__ queue_counter,
                                                          unsigned long long* __restrict__ counters) {
-    wave_loop<kPrec, false>(ManySources{sources}, entries, n_entries, chunk, queue_counter, counters);
+    wave_loop<kPrec, kMergeDrain>(ManySources{sources}, entries, n_entries, chunk, queue_counter, counters);
 }
 
 __global__ void k_zero_words(unsigned* p, int n) {
@@ -810,19 +810,36 @@ hipError_t launch_deposit_tiles(hipStream_t stream, int precision, const TileSou
 hipError_t launch_deposit_step(hipStream_t stream, int precision, const TileSource* d_sources, const int2* d_entries,
                                int n_entries, unsigned* queue_counter, unsigned long long* counters, int device) {
     if (n_entries <= 0) return hipSuccess;
-    const int chunk = std::max(1, launch_shape().chunk);
+    // small steps as small launches (launch_deposit_tiles)
+    const int64_t resident_waves = int64_t(resident_blocks(device)) * kWavesPerBlock;
+    const int chunk = int(std::max<int64_t>(1, std::min<int64_t>(launch_shape().chunk, n_entries / resident_waves)));
+    const bool merge = n_entries < kMergeDrainTiles * resident_waves;
     const int64_t waves_needed = (int64_t(n_entries) + chunk - 1) / chunk;
     const unsigned grid = unsigned(std::max<int64_t>(1, std::min<int64_t>((waves_needed + kWavesPerBlock - 1) / kWavesPerBlock,
                                                                           resident_blocks(device))));
-    if (precision == ADECO_PREC_FP64)
-        hipLaunchKernelGGL(k_deposit_step<ADECO_PREC_FP64>, dim3(grid), dim3(kBlock), 0, stream, d_sources, d_entries, n_entries,
-                           chunk, queue_counter, counters);
-    else if (precision == ADECO_PREC_MIXED)
-        hipLaunchKernelGGL(k_deposit_step<ADECO_PREC_MIXED>, dim3(grid), dim3(kBlock), 0, stream, d_sources, d_entries, n_entries,
-                           chunk, queue_counter, counters);
-    else
-        hipLaunchKernelGGL(k_deposit_step<ADECO_PREC_FP32>, dim3(grid), dim3(kBlock), 0, stream, d_sources, d_entries, n_entries,
-                           chunk, queue_counter, counters);
+    const dim3 g3(grid), b3(kBlock);
+    if (precision == ADECO_PREC_FP64) {
+        if (merge)
+            hipLaunchKernelGGL((k_deposit_step<ADECO_PREC_FP64, true>), g3, b3, 0, stream, d_sources, d_entries, n_entries,
+                               chunk, queue_counter, counters);
+        else
+            hipLaunchKernelGGL((k_deposit_step<ADECO_PREC_FP64, false>), g3, b3, 0, stream, d_sources, d_entries, n_entries,
+                               chunk, queue_counter, counters);
+    } else if (precision == ADECO_PREC_MIXED) {
+        if (merge)
+            hipLaunchKernelGGL((k_deposit_step<ADECO_PREC_MIXED, true>), g3, b3, 0, stream, d_sources, d_entries, n_entries,
+                               chunk, queue_counter, counters);
+        else
+            hipLaunchKernelGGL((k_deposit_step<ADECO_PREC_MIXED, false>), g3, b3, 0, stream, d_sources, d_entries, n_entries,
+                               chunk, queue_counter, counters);
+    } else {
+        if (merge)
+            hipLaunchKernelGGL((k_deposit_step<ADECO_PREC_FP32, true>), g3, b3, 0, stream, d_sources, d_entries, n_entries,
+                               chunk, queue_counter, counters);
+        else
+            hipLaunchKernelGGL((k_deposit_step<ADECO_PREC_FP32, false>), g3, b3, 0, stream, d_sources, d_entries, n_entries,
+                               chunk, queue_counter, counters);
+    }
     return hipGetLastError();
 }
 
