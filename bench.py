@@ -13,6 +13,12 @@ mixed-precision form with its error against FP64, the total-mass drift, the roof
 dominant kernel (HIP events on its launch stream) and a CPU baseline: the compiled reference
 CPU path (oracle/_ref, kind "reference") or the oracle restatement (kind "port") timed on a
 bounded sample of the same leaves on this host's cores.
+
+Studies, one JSON line each instead of the headline: --heavy-tail (256^3 lattice with infall
+centres, batched vs plain), --precision (precision modes and near-degenerate tets), --serve (ROI
+requests, continuous batching vs serial plain), --program (the whole program from a snapshot file
+to leaf files, SORTED_BY_ID and DARK_SKY, beside the reference program) and --boundary (the drop-in
+deposit_tets called per leaf, plain vs batched form).
 """
 
 from __future__ import annotations
