@@ -109,6 +109,9 @@ DARK_SKY_HEADERS = {
     "no_semicolon": ("int sha1_chunks = 1\n# SDF-EOH more text\n", 1),
     "no_end_of_header": ("# SDF 1.0\nint sha1_chunks = 2;\n", 0),
     "no_sha1_chunks": ("# SDF 1.0\n# SDF-EOH\n", 0),
+    # a line of 1023 characters fits the reader's 1024-byte line buffer; one of 1024 does not
+    "line_1023": ("# SDF 1.0\n#" + "x" * 1022 + "\nint sha1_chunks = 2;\n# SDF-EOH\n", 2),
+    "line_1024": ("# SDF 1.0\n#" + "x" * 1023 + "\nint sha1_chunks = 2;\n# SDF-EOH\n", 2),
 }
 
 
