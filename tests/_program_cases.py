@@ -43,6 +43,12 @@ PARAM_TEXTS = {
     "roi_five_values": "region_of_interest = 1 2 3 4 5\n",
     "bad_number": "stride = abc\n",
     "bad_roi_number": "region_of_interest = 1 2 x 4 5 6\n",
+    # at most 10000 line reads, the one that meets the end of the file included: a file of 9998
+    # lines is read, one of 9999 is rejected
+    "lines_9998": "\n" * 9996 + "stride = 2\nmax_refinement_level = 9\n",
+    "lines_9999": "\n" * 9997 + "stride = 2\nmax_refinement_level = 9\n",
+    # without '=' the whole line is both key and value
+    "no_equals": "use_region_of_interest true\ninput_file /a/b.bin\n",
 }
 
 # Whole runs of the program: a snapshot (n particles per axis, rms displacement in spacings,

@@ -65,6 +65,8 @@ def params_fixture(r, tmp):
         with open(path, "w") as f:
             f.write(text)
         res[name] = ref_parse(r, path)
+        if "error" in res[name]:
+            res[name]["error"] = res[name]["error"].replace(path, "<file>")
     res["missing_file"] = ref_parse(r, os.path.join(tmp, "no_such_file.txt"))
     res["missing_file"]["error"] = res["missing_file"]["error"].replace(tmp, "<dir>")
     return res
