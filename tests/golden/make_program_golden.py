@@ -92,7 +92,13 @@ def reader_fixture(r, tmp):
     np.zeros(13, np.uint8).tofile(odd)
     noncubic = os.path.join(tmp, "noncubic.bin")
     np.zeros(30, np.float32).tofile(noncubic)
-    out = {"good": ref_load(r, good), "odd_size": ref_load(r, odd), "not_cubic": ref_load(r, noncubic)}
+    empty = os.path.join(tmp, "empty.bin")
+    open(empty, "wb").close()
+    out = {"good": ref_load(r, good), "odd_size": ref_load(r, odd), "not_cubic": ref_load(r, noncubic),
+           "empty": ref_load(r, empty), "missing": ref_load(r, os.path.join(tmp, "no_such_file.bin"))}
+    for key in ("empty", "missing"):
+        if "error" in out[key]:
+            out[key]["error"] = out[key]["error"].replace(tmp, "<dir>")
     out["good"]["snapshot"] = [6, 1.0, 2]
     return out
 

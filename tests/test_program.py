@@ -121,6 +121,14 @@ def test_sorted_by_id_reader_matches_reference(tmp_path):
         with pytest.raises(A.AdRuntimeException) as e:
             get_reader("SORTED_BY_ID").load_data(str(tmp_path / fname))
         assert ref_message(g[key]["error"]) in str(e.value)
+    # an empty file is an empty lattice for the reader (the program rejects it later)
+    (tmp_path / "empty.bin").write_bytes(b"")
+    x, y, z, dims = get_reader("SORTED_BY_ID").load_data(str(tmp_path / "empty.bin"))
+    assert list(dims) == g["empty"]["dims"] and x.size == g["empty"]["n"] == 0
+    assert sha(np.concatenate([x, y, z])) == g["empty"]["xyz_sha"]
+    with pytest.raises(A.AdRuntimeException) as e:
+        get_reader("SORTED_BY_ID").load_data(str(tmp_path / "no_such_file.bin"))
+    assert ref_message(g["missing"]["error"]).replace("<dir>", str(tmp_path)) in str(e.value)
     with pytest.raises(A.AdRuntimeException, match="invalid 'stride'"):
         get_reader("SORTED_BY_ID").load_data(str(path), stride=0)
     with pytest.raises(A.AdRuntimeException, match="not supported"):
