@@ -41,6 +41,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "../../include/adeco.h"
@@ -715,13 +716,38 @@ __global__ void k_zero_words(unsigned* p, int n) {
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-int resident_blocks(int device) {
+// Resident workgroups of a launch: blocks_per_cu per CU, capped at what the kernel's registers
+// and LDS let a CU hold (a surplus workgroup would only start once a resident one had drained the
+// work queue).
+int resident_blocks(int device, int precision, bool merge) {
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-    return cus * std::max(1, launch_shape().blocks_per_cu);
+    return cus * std::max(1, std::min(launch_shape().blocks_per_cu, deposit_occupancy(precision, merge)));
 }
 
 }  // namespace
+
+int deposit_occupancy(int precision, bool merge) {
+    // per (precision, merge) instantiation of k_deposit_waves; k_deposit_step shares its resources
+    static std::atomic<int> cache[3][2];
+    const int pi = precision == ADECO_PREC_FP64 ? 0 : (precision == ADECO_PREC_MIXED ? 1 : 2);
+    std::atomic<int>& n = cache[pi][merge ? 1 : 0];
+    const int have = n.load(std::memory_order_relaxed);
+    if (have > 0) return have;
+    int got = 0;
+    hipError_t e = hipErrorInvalidValue;
+    const void* f = nullptr;
+    if (pi == 0) f = merge ? reinterpret_cast<const void*>(k_deposit_waves<ADECO_PREC_FP64, true>)
+                           : reinterpret_cast<const void*>(k_deposit_waves<ADECO_PREC_FP64, false>);
+    else if (pi == 1) f = merge ? reinterpret_cast<const void*>(k_deposit_waves<ADECO_PREC_MIXED, true>)
+                                : reinterpret_cast<const void*>(k_deposit_waves<ADECO_PREC_MIXED, false>);
+    else f = merge ? reinterpret_cast<const void*>(k_deposit_waves<ADECO_PREC_FP32, true>)
+                   : reinterpret_cast<const void*>(k_deposit_waves<ADECO_PREC_FP32, false>);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&got, f, kBlock, 0);
+    const int v = (e == hipSuccess && got > 0) ? got : 1;
+    n.store(v, std::memory_order_relaxed);  // concurrent first calls store the same value
+    return v;
+}
 
 size_t candidate_space_bytes(int64_t n_pairs) {
     const int64_t nblk = (n_pairs + kScanBlock - 1) / kScanBlock;
@@ -781,12 +807,14 @@ hipError_t launch_deposit_tiles(hipStream_t stream, int precision, const TileSou
     if (src.n_tiles <= 0) return hipSuccess;
     // small launches: fewer tiles per pop when there are too few to give every resident wave
     // `chunk` of them, and drain passes that merge the work classes
-    const int64_t resident_waves = int64_t(resident_blocks(device)) * kWavesPerBlock;
+    // (the merged-drain instantiation decides with the large one's occupancy; it is the same or
+    // lower, and resident_blocks caps the grid with the chosen instantiation's own)
+    const int64_t resident_waves = int64_t(resident_blocks(device, precision, false)) * kWavesPerBlock;
     const int chunk = int(std::max<int64_t>(1, std::min<int64_t>(launch_shape().chunk, src.n_tiles / resident_waves)));
     const bool merge = src.n_tiles < kMergeDrainTiles * resident_waves;
     const int64_t waves_needed = (src.n_tiles + chunk - 1) / chunk;
     const unsigned grid = unsigned(std::max<int64_t>(1, std::min<int64_t>((waves_needed + kWavesPerBlock - 1) / kWavesPerBlock,
-                                                                          resident_blocks(device))));
+                                                                          resident_blocks(device, precision, merge))));
     const dim3 g3(grid), b3(kBlock);
     if (precision == ADECO_PREC_FP64) {
         if (merge)
@@ -811,12 +839,12 @@ hipError_t launch_deposit_step(hipStream_t stream, int precision, const TileSour
                                int n_entries, unsigned* queue_counter, unsigned long long* counters, int device) {
     if (n_entries <= 0) return hipSuccess;
     // small steps as small launches (launch_deposit_tiles)
-    const int64_t resident_waves = int64_t(resident_blocks(device)) * kWavesPerBlock;
+    const int64_t resident_waves = int64_t(resident_blocks(device, precision, false)) * kWavesPerBlock;
     const int chunk = int(std::max<int64_t>(1, std::min<int64_t>(launch_shape().chunk, n_entries / resident_waves)));
     const bool merge = n_entries < kMergeDrainTiles * resident_waves;
     const int64_t waves_needed = (int64_t(n_entries) + chunk - 1) / chunk;
     const unsigned grid = unsigned(std::max<int64_t>(1, std::min<int64_t>((waves_needed + kWavesPerBlock - 1) / kWavesPerBlock,
-                                                                          resident_blocks(device))));
+                                                                          resident_blocks(device, precision, merge))));
     const dim3 g3(grid), b3(kBlock);
     if (precision == ADECO_PREC_FP64) {
         if (merge)

@@ -78,6 +78,9 @@ struct LaunchShape {
     int blocks_per_cu = 4;
 };
 LaunchShape& launch_shape();
+// Workgroups of the persistent deposit kernel (precision, merged drain or not) that one CU holds at
+// once (hipOccupancyMaxActiveBlocksPerMultiprocessor on the current device); at least 1.
+int deposit_occupancy(int precision, bool merge);
 
 // Bytes of the work slab for n_pairs pairs (tile map excluded: it needs n_cand).
 size_t candidate_space_bytes(int64_t n_pairs);

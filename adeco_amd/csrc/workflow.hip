@@ -405,12 +405,18 @@ adeco_status adeco_autotune_batched(adeco_octree T, int dims, int decomposition,
                                     int* best_chunk, int* best_blocks_per_cu, double* best_ms) {
     if (!T || reps < 1) return fail(ADECO_EINVAL, "autotune_batched(): null handle or reps < 1");
     static const int chunks[] = {1, 2, 4, 8, 16, 32};
-    static const int blocks[] = {1, 2, 3, 4};
+    // workgroups per CU up to what the kernel's resources let a CU hold (at most 4)
+    int max_blocks = 1;
+    {
+        std::lock_guard<std::mutex> lk(T->L->ds->mutex);
+        if (hipSetDevice(T->L->device) != hipSuccess) return fail(ADECO_EDEVICE, "autotune_batched(): hipSetDevice failed");
+        max_blocks = std::min(4, deposit_occupancy(precision, false));
+    }
     const LaunchShape saved = launch_shape();
     double best = 1e300;
     LaunchShape pick = saved;
     for (int c : chunks)
-        for (int b : blocks) {
+        for (int b = 1; b <= max_blocks; ++b) {
             launch_shape().chunk = c;
             launch_shape().blocks_per_cu = b;
             double t = 1e300;

@@ -1,18 +1,22 @@
 """Throughput of the exact tetrahedron mass deposit on one MI355X (tets/s), one JSON line.
 
-Workload (the 128^3 plain-GPU configuration): a 128^3-particle lattice (129^3 unwrapped vertices,
-12,582,912 tets in the Kuhn decomposition, tet mass 1/6 so a particle carries mass 1) with a
-seeded k^-2 Gaussian displacement field at rms 3 lattice spacings; the reference's adaptive
-octree (max depth 6, split above 50,000 estimated tets) with 16^3-cell leaves.  A step is one
-deposit of every lattice tet into every leaf it overlaps, from positions resident in HBM to FP64
-leaf grids in HBM (octree and binning are built once, outside the timed region, and timed
-separately).  tets/s = 6 N^3 / step time; each tet counts once however many leaves it touches.
+Workload (the heavy-tailed batched configuration, the largest that fits one GPU): a 256^3-particle
+lattice (257^3 unwrapped vertices, 100,663,296 tets in the Kuhn decomposition, tet mass 1/6 so a
+particle carries mass 1) with a seeded k^-2 Gaussian displacement field at rms 4 lattice spacings
+plus 64 seeded spherical infall centres (radius 2-8 Mpc/h) that pull tets into caustics; the
+reference's adaptive octree (max depth 6, split above 50,000 estimated tets) with 16^3-cell leaves
+(`value`) and, as an extra field, 32^3-cell leaves.  A step is the per-leaf binning of the lattice
+plus the deposit of every lattice tet into every leaf it overlaps, from positions resident in HBM to
+FP64 leaf grids in HBM (the octree is built once, outside the timed region, and timed separately).
+tets/s = 6 N^3 / step time; each tet counts once however many leaves it touches.
+`--workload c2` runs the same measurement on the 128^3 lattice (rms 3 spacings, no infall).
 
 Extra fields: the plain one-tet-per-thread per-leaf-launch form measured the same way, the
-mixed-precision form with its error against FP64, the total-mass drift, the roofline of the
-dominant kernel (HIP events on its launch stream) and a CPU baseline: the compiled reference
-CPU path (oracle/_ref, kind "reference") or the oracle restatement (kind "port") timed on a
-bounded sample of the same leaves on this host's cores.
+hipGraph replay, the whole one-rank workflow, the total-mass drift, the roofline of the dominant
+kernel (HIP events on its launch stream; algorithmic FLOPs and bytes from the oracle's counts in
+tests/golden/c{2,3}_flops.json) and a CPU baseline: the compiled reference CPU path (oracle/_ref,
+kind "reference") or the oracle restatement (kind "port") timed on a bounded sample of the same
+leaves on this host's cores, and on one core.
 
 Studies, one JSON line each instead of the headline: --heavy-tail (256^3 lattice with infall
 centres, batched vs plain), --precision (precision modes and near-degenerate tets), --serve (ROI
@@ -38,18 +42,27 @@ sys.path.insert(0, ROOT)
 import adeco_amd as A  # noqa: E402
 from adeco_amd.lattice import make_lattice  # noqa: E402
 
+# the two adaptive workloads: lattice cubes per axis, rms displacement (spacings), seed, infall
+# centres, flops fixture
+WORKLOADS = {
+    "c3": {"n": 256, "rms": 4.0, "seed": 3, "infall": 64, "flops": "c3_flops.json"},
+    "c2": {"n": 128, "rms": 3.0, "seed": 2, "infall": 0, "flops": "c2_flops.json"},
+}
 N = 128
 RMS = 3.0
 SEED = 2
 MAX_TETS = 50000
 MAX_LEVEL = 6
 DIMS = 16
+DIMS_FINE = 32
 TET_MASS = 1.0 / 6.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (the roof of an FP64 VALU-bound kernel)
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of k_deposit_waves<FP64> on this workload: rocprofv3 FETCH_SIZE x 2 +
-# WRITE_SIZE x 1 (each scaled by what it reads for a 1 GiB copy), profiles/deposit_waves_c2_pmc.md
-TRAFFIC_BYTES = 5.526e9
+# HBM bytes per launch of k_deposit_waves<FP64, false> on the 256^3 infall workload at 16^3-cell
+# leaves: rocprofv3 FETCH_SIZE + WRITE_SIZE (KiB per dispatch, each scaled by what it reads for a
+# 1 GiB copy), profiles/deposit_waves_c3_pmc.md
+TRAFFIC_BYTES = {"c3": None, "c2": None}
 
 
 def log(*a):
@@ -92,80 +105,153 @@ def _stdout_to_stderr(fn):
         os.close(saved)
 
 
-def cpu_baseline(lat, tree_leaves_order, budget_s=20.0):
+def host_cpu_info():
+    """This host's CPUs: nproc, the model name (lscpu), the affinity set and the cgroup CPU quota
+    (cpu.max), and the threads a CPU baseline may use (the smaller of affinity and quota)."""
+    import subprocess
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = nproc
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    threads = aff if quota is None else max(1, min(aff, int(quota)))
+    return {"nproc": nproc, "cpu_model": model, "affinity_cpus": aff, "cgroup_cpu_quota": quota, "threads": threads}
+
+
+def cpu_baseline(lat, dims, budget_s=20.0, one_core_s=10.0):
     """Times the reference CPU path (or the oracle port) on this host's cores; returns a dict.
 
-    The reference deposits the leaves in its processing order, one leaf per worker thread at a
-    time, each thread into its own float grid, until every leaf is done or budget_s has passed;
-    a partial run is extrapolated by the meta-patch share of the leaves it finished."""
+    The compiled reference deposits leaves as its worker threads do (leaf gather +
+    DepositWrapper::deposit_tets into a float grid per thread, REF_EFFECTIVE tets), one leaf per
+    thread at a time, in a seeded random leaf order, until every leaf is done or the time budget
+    has passed; a partial run is extrapolated to the whole job by the meta-patch share of the leaves
+    it finished.  Run once on all the threads this host gives the process and once on one thread."""
     import threading
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
     from tests import _oracle as O  # checker side only
     from tests._octree_io import ref_octree
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu_info()
+    threads = host["threads"]
     ref = O.load_ref()
     if ref is not None:
         t_tree = time.perf_counter()
         d = _stdout_to_stderr(lambda: ref_octree(lat, MAX_TETS, MAX_LEVEL))
         t_tree = time.perf_counter() - t_tree
-        order = [int(x) for x in d["order"]]
         dfs = list(d["dfs"])
         n_meta = np.diff(d["mc_offsets"])
         meta_of = {int(node): int(n_meta[i]) for i, node in enumerate(dfs)}
-        lock = threading.Lock()
-        todo = iter(order)
-        done = []
-        t0 = time.perf_counter()
+        order = [int(x) for x in np.random.default_rng(0).permutation(np.asarray(d["order"]))]
 
-        def worker():
-            g = np.zeros(DIMS ** 3, np.float32)
-            while time.perf_counter() - t0 < budget_s:
-                with lock:
-                    node = next(todo, None)
-                if node is None:
-                    return
-                g[:] = 0
-                assert ref.ref_octree_deposit_leaf(node, DIMS, O.P(g), None, None, None) == 0
-                with lock:
-                    done.append(node)
+        def run(nthreads, budget):
+            lock = threading.Lock()
+            todo = iter(order)
+            done = []
+            t0 = time.perf_counter()
 
-        pool = [threading.Thread(target=worker) for _ in range(threads)]
-        for th in pool:
-            th.start()
-        for th in pool:
-            th.join()
-        wall = time.perf_counter() - t0
-        frac = sum(meta_of[n] for n in done) / float(sum(meta_of.values()))
-        est_wall = wall / frac
-        whole = len(done) == len(order)
-        return {"value": lat.num_tets / est_wall, "unit": "tets/s", "cores": threads, "kind": "reference",
-                "sample": (f"{'all' if whole else len(done)} of {len(order)} leaves"
+            def worker():
+                g = np.zeros(dims ** 3, np.float32)
+                while time.perf_counter() - t0 < budget:
+                    with lock:
+                        node = next(todo, None)
+                    if node is None:
+                        return
+                    g[:] = 0
+                    assert ref.ref_octree_deposit_leaf(node, dims, O.P(g), None, None, None) == 0
+                    with lock:
+                        done.append(node)
+
+            pool = [threading.Thread(target=worker) for _ in range(nthreads)]
+            for th in pool:
+                th.start()
+            for th in pool:
+                th.join()
+            wall = time.perf_counter() - t0
+            frac = sum(meta_of[n] for n in done) / float(sum(meta_of.values()))
+            return wall, len(done), frac
+
+        wall, ndone, frac = _stdout_to_stderr(lambda: run(threads, budget_s))
+        wall1, ndone1, frac1 = _stdout_to_stderr(lambda: run(1, one_core_s))
+        whole = ndone == len(order)
+        return {"value": lat.num_tets / (wall / frac), "unit": "tets/s", "cores": threads, "kind": "reference",
+                "one_core_value": lat.num_tets / (wall1 / frac1), **{k: host[k] for k in host if k != "threads"},
+                "sample": (f"{'all' if whole else ndone} of {len(order)} leaves"
                            f"{'' if whole else f' ({frac:.3f} of the meta-patch work, whole-job time extrapolated by that share)'}"
-                           f" deposited by the compiled reference DepositWrapper::deposit_tets (REF_EFFECTIVE tets, "
-                           f"float grids, leaf gather included) on {threads} threads in {wall:.2f} s; the reference "
-                           f"octree (generate_tree, one thread) took {t_tree:.2f} s before it, not included"),
+                           f" in a seeded random order, deposited at {dims}^3 cells by the compiled reference (its leaf "
+                           f"gather + DepositWrapper::deposit_tets, REF_EFFECTIVE tets, float grids) on {threads} threads "
+                           f"in {wall:.2f} s; one thread: {ndone1} leaves ({frac1:.4f} of the work) in {wall1:.2f} s; the "
+                           f"reference octree (generate_tree, one thread) took {t_tree:.2f} s before it, not included"),
                 "octree_s": t_tree}
     # fallback: the oracle port, FP64, on a fixed tet sample of the whole lattice
     box = np.asarray(A.cubic_root_box(lat.bbox()))
     ids = np.arange(0, lat.num_tets, 97, dtype=np.int32)
     t0 = time.perf_counter()
-    O.deposit_lattice(lat, box, DIMS * 8, decomp=2, tet_mass=TET_MASS, tet_ids=ids, precision=1, nthreads=threads)
+    O.deposit_lattice(lat, box, dims * 8, decomp=2, tet_mass=TET_MASS, tet_ids=ids, precision=1, nthreads=threads)
     wall = time.perf_counter() - t0
     return {"value": ids.size / wall, "unit": "tets/s", "cores": threads, "kind": "port",
-            "sample": f"{ids.size} tets (every 97th) of the lattice on one 128^3 grid over the root box, FP64, "
-                      f"{threads} threads, {wall:.2f} s"}
+            **{k: host[k] for k in host if k != "threads"},
+            "sample": f"{ids.size} tets (every 97th) of the lattice on one grid of {dims * 8}^3 cells over the root box, "
+                      f"FP64, {threads} threads, {wall:.2f} s"}
 
 
-def algorithmic_flops():
-    """Reference-algorithm FP64 operations of one step of this workload, counted once by the
-    oracle's counting instantiation and committed as tests/golden/c2_flops.json (written by
-    tests/golden/make_flops.py); returns (flops, pairs) or None when the fixture is absent."""
-    path = os.path.join(ROOT, "tests", "golden", "c2_flops.json")
+def algorithmic_work(workload):
+    """Reference-algorithm FP64 operations and nonzero (tet, cell) contributions of one step,
+    counted by the oracle's counting instantiation (tests/golden/make_flops.py) and committed as
+    tests/golden/c{2,3}_flops.json; None when the fixture is absent.  Returns {dims: {"flops",
+    "nonzero", "nonzero_per_pair_by_level"}}."""
+    path = os.path.join(ROOT, "tests", "golden", WORKLOADS[workload]["flops"])
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return float(d["flops"]), int(d["pairs"])
+    if "dims" not in d:  # every leaf counted, 16^3 cells
+        return {16: {"flops": float(d["flops"]), "nonzero": d.get("nonzero_contributions"), "by_level": None,
+                     "flops_rel_std_err": 0.0, "pairs": int(d["pairs"])}}
+    return {int(k): {"flops": float(v["flops"]), "nonzero": float(v["nonzero_contributions"]),
+                     "by_level": {int(l): r["nonzero_per_pair"] for l, r in v["per_level"].items()},
+                     "flops_rel_std_err": float(v["flops_rel_std_err"]), "pairs": int(d["pairs"])}
+            for k, v in d["dims"].items()}
+
+
+def roofline_of(work, dims, kernel_ms, lat, n_leaves, counts=None, levels=None, traffic=None):
+    """The dominant kernel's roofline: algorithmic FLOPs / its HIP-event duration against the FP64
+    vector peak, and algorithmic bytes (12 B per lattice vertex, 8 B per nonzero (tet, cell)
+    contribution - the FP64 accumulation - and the leaf grids once) against the HBM peak.  With the
+    run's per-leaf pair counts and leaf levels, the nonzero contributions are the fixture's rate per
+    pair at each octree level times this run's pairs at that level."""
+    w = work.get(dims) if work else None
+    if w is None:
+        return None
+    achieved = w["flops"] / (kernel_ms * 1e-3) / 1e12
+    nonzero = w["nonzero"]
+    if w["by_level"] and counts is not None and levels is not None:
+        nonzero = float(sum(w["by_level"].get(int(l), 0.0) * float(counts[levels == l].sum()) for l in np.unique(levels)))
+    nv = float(np.prod(lat.dims))
+    alg_bytes = None if nonzero is None else 12.0 * nv + 8.0 * nonzero + 8.0 * n_leaves * dims ** 3
+    rec = {"bound": "valu", "pipe": "VALU FP64 (clipping geometry; no matrix work)",
+           "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+           "frac_of_fp32_peak": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+           "flops_per_launch": w["flops"], "flops_rel_std_err": w["flops_rel_std_err"], "kernel_ms": kernel_ms,
+           "kernel": "k_deposit_waves<FP64>", "leaf_dims": dims}
+    if alg_bytes is not None:
+        rec.update({"algorithmic_bytes": alg_bytes, "nonzero_contributions": nonzero,
+                    "hbm_achieved_gbs": alg_bytes / (kernel_ms * 1e-3) / 1e9,
+                    "hbm_frac": alg_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+    return rec
 
 
 SERVE_N = 256
@@ -542,7 +628,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--quick", action="store_true", help="skip the plain and mixed comparison runs")
+    ap.add_argument("--quick", action="store_true",
+                    help="skip the 32^3-leaf, plain-kernel and reference-decomposition comparison runs")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c3",
+                    help="c3: 256^3 lattice with infall (default); c2: 128^3 lattice")
     ap.add_argument("--shape", default="", help="launch shape CHUNK,WORKGROUPS_PER_CU instead of the autotuner's")
     ap.add_argument("--serve", action="store_true", help="run only the ROI serving study (256^3 lattice)")
     ap.add_argument("--precision", action="store_true", help="run only the precision-mode study")
@@ -590,10 +679,19 @@ def main():
                           **res}))
         return
 
+    headline(args)
+
+
+def headline(args):
+    wl = WORKLOADS[args.workload]
+    n = wl["n"]
     t0 = time.perf_counter()
-    lat = make_lattice(N, RMS, seed=SEED)
-    log(f"[bench] lattice {N}^3 generated in {time.perf_counter() - t0:.1f}s")
+    lat = make_lattice(n, wl["rms"], seed=wl["seed"], infall_centres=wl["infall"])
+    log(f"[bench] lattice {n}^3 ({args.workload}) generated in {time.perf_counter() - t0:.1f}s")
     decomp = A.DECOMP_KUHN
+    work = algorithmic_work(args.workload)
+    if work is None:
+        log(f"[bench] roofline skipped: tests/golden/{wl['flops']} is missing")
     with A.ResidentLattice.from_lattice(lat) as rl:
         t0 = time.perf_counter()
         tree = A.Octree(rl, MAX_TETS, MAX_LEVEL)
@@ -602,8 +700,11 @@ def main():
         counts = tree.bin(decomp)
         t_bin = time.perf_counter() - t0
         leaves = tree.leaves()["dfs"]
+        levels = tree.nodes()["level"][leaves]
+        nz = counts[counts > 0]
         log(f"[bench] octree {tree.num_nodes} nodes / {tree.num_leaves} leaves in {t_oct*1e3:.1f} ms; "
-            f"binning {counts.sum()} pairs in {t_bin*1e3:.1f} ms")
+            f"binning {counts.sum()} pairs in {t_bin*1e3:.1f} ms; per-leaf tets min/median/max "
+            f"{nz.min()}/{int(np.median(nz))}/{nz.max()}")
 
         # launch shape by the autotuner (untimed) or --shape, then the headline: batched persistent
         # form, FP64
@@ -621,21 +722,24 @@ def main():
         value = lat.num_tets / (wall / args.steps)
         grids64 = tree.leaf_grids()
         total = float(grids64.sum())
-        drift = abs(total - N ** 3) / N ** 3
-        log(f"[bench] batched fp64: {step_ms:.2f} ms/step, kernel {main_ms:.2f} ms, {value:.3e} tets/s, "
+        drift = abs(total - n ** 3) / n ** 3
+        log(f"[bench] batched fp64 {DIMS}^3 leaves: {step_ms:.2f} ms/step, kernel {main_ms:.2f} ms, {value:.3e} tets/s, "
             f"candidates {tl['candidates']}, partials {tl['partials']}, exact recomputes {tl['fallbacks']}, "
             f"mass drift {drift:.2e}")
+        roofline = roofline_of(work, DIMS, main_ms, lat, tree.num_leaves, counts, levels, TRAFFIC_BYTES[args.workload])
 
-        extra = {}
+        extra = {"exact_recomputes": tl["fallbacks"], "partial_candidates": tl["partials"], "candidates": tl["candidates"]}
         # the same deposit captured as a hipGraph and replayed (candidate space built once)
         with A.DepositGraph(tree, DIMS, decomp, TET_MASS, A.PREC_FP64) as g:
             g.launch()
             gms = [g.launch() for _ in range(max(1, args.steps))]
             gerr = max_rel_err(g.leaf_grids(), grids64, 1e-8 * TET_MASS)
+        del grids64
         extra["graph_replay_ms"] = float(np.mean(gms))
         extra["graph_replay_tets_per_s"] = lat.num_tets / (float(np.mean(gms)) * 1e-3)
         extra["graph_vs_eager_max_rel_err"] = gerr
-        # the whole one-rank workflow per step: octree refinement + binning + deposit
+        log(f"[bench] graph replay: {np.mean(gms):.2f} ms ({extra['graph_replay_tets_per_s']:.3e} tets/s), err {gerr:.1e}")
+        # the whole one-rank workflow per step: octree refinement + binning + deposit + leaf grids to host
         wf = []
         for _ in range(3):
             t0 = time.perf_counter()
@@ -648,48 +752,50 @@ def main():
         extra["workflow_tets_per_s"] = lat.num_tets / float(np.min(wf))
         log(f"[bench] workflow (octree + binning + deposit + leaf grids to host): {extra['workflow_ms_per_step']:.2f} ms "
             f"({extra['workflow_tets_per_s']:.3e} tets/s)")
-        log(f"[bench] graph replay: {np.mean(gms):.2f} ms ({extra['graph_replay_tets_per_s']:.3e} tets/s), err {gerr:.1e}")
         if not args.quick:
-            wall_p, main_p, _ = run_steps(tree, max(2, args.steps // 2), 1, A.MODE_PLAIN, A.PREC_FP64, decomp)
-            plain_tets = lat.num_tets / (wall_p / max(2, args.steps // 2))
+            # 32^3-cell leaves: the same lattice and octree, its own launch shape
+            fine_shape = shape if args.shape else tree.autotune(DIMS_FINE, decomp, TET_MASS, A.PREC_FP64, reps=1)
+            steps_f = max(2, args.steps // 3)
+            wall_f, main_f, tlf = run_steps(tree, steps_f, 1, A.MODE_BATCHED, A.PREC_FP64, decomp, DIMS_FINE)
+            fine = tree.leaf_grids()
+            extra["fine_leaves"] = {
+                "leaf_dims": DIMS_FINE, "tets_per_s": lat.num_tets / (wall_f / steps_f), "ms_per_step": wall_f / steps_f * 1e3,
+                "kernel_ms": main_f, "launch_shape": fine_shape, "mass_drift": abs(float(fine.sum()) - n ** 3) / n ** 3,
+                "candidates": tlf["candidates"], "partial_candidates": tlf["partials"],
+                "roofline": roofline_of(work, DIMS_FINE, main_f, lat, tree.num_leaves, counts, levels)}
+            del fine
+            log(f"[bench] batched fp64 {DIMS_FINE}^3 leaves: {wall_f / steps_f * 1e3:.2f} ms/step, kernel {main_f:.2f} ms "
+                f"({extra['fine_leaves']['tets_per_s']:.3e} tets/s)")
+            A.set_launch_shape(shape["chunk"], shape["blocks_per_cu"])
+            # the plain one-tet-per-thread form, one launch per leaf
+            steps_p = 1 if n > 128 else max(2, args.steps // 2)
+            wall_p, main_p, _ = run_steps(tree, steps_p, 0 if n > 128 else 1, A.MODE_PLAIN, A.PREC_FP64, decomp)
+            plain_tets = lat.num_tets / (wall_p / steps_p)
             plain_grids = tree.leaf_grids()
+            tree.deposit(DIMS, decomposition=decomp, tet_mass=TET_MASS, mode=A.MODE_BATCHED, precision=A.PREC_FP64)
             extra["plain_tets_per_s"] = plain_tets
             extra["batched_over_plain"] = value / plain_tets
-            extra["plain_vs_batched_max_rel_err"] = max_rel_err(plain_grids, grids64, 1e-8 * TET_MASS)
-            log(f"[bench] plain fp64: {wall_p / max(2, args.steps // 2) * 1e3:.2f} ms/step, {plain_tets:.3e} tets/s")
-            wall_m, main_m, tlm = run_steps(tree, max(2, args.steps // 2), 1, A.MODE_BATCHED, A.PREC_MIXED, decomp)
-            mixed_grids = tree.leaf_grids()
-            extra["mixed_tets_per_s"] = lat.num_tets / (wall_m / max(2, args.steps // 2))
-            extra["mixed_max_rel_err_vs_fp64"] = max_rel_err(mixed_grids, grids64, 1e-8 * TET_MASS)
-            extra["mixed_fp64_recompute_fraction"] = tlm["fallbacks"] / max(tlm["partials"], 1)
-            log(f"[bench] mixed: {extra['mixed_tets_per_s']:.3e} tets/s, err {extra['mixed_max_rel_err_vs_fp64']:.2e}, "
-                f"recompute {extra['mixed_fp64_recompute_fraction']:.3f}")
+            extra["plain_vs_batched_max_rel_err"] = max_rel_err(plain_grids, tree.leaf_grids(), 1e-8 * TET_MASS)
+            del plain_grids
+            log(f"[bench] plain fp64: {wall_p / steps_p * 1e3:.2f} ms/step, {plain_tets:.3e} tets/s, "
+                f"batched/plain {extra['batched_over_plain']:.1f}x, max rel diff {extra['plain_vs_batched_max_rel_err']:.1e}")
             # the reference's own (non-tiling) decomposition, like for like with the CPU baseline
+            tree.bin(A.DECOMP_REF_EFFECTIVE)
             wall_r, _, _ = run_steps(tree, max(2, args.steps // 2), 1, A.MODE_BATCHED, A.PREC_FP64, A.DECOMP_REF_EFFECTIVE)
             extra["ref_effective_tets_per_s"] = lat.num_tets / (wall_r / max(2, args.steps // 2))
             tree.bin(decomp)
 
-        roofline = None
-        fl = algorithmic_flops()
-        if fl is not None:
-            achieved = fl[0] / (main_ms * 1e-3) / 1e12
-            roofline = {"bound": "mfma", "pipe": "VALU FP64 (clipping geometry; no matrix work)",
-                        "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": TRAFFIC_BYTES,
-                        "flops_per_launch": fl[0], "pairs_counted": fl[1], "kernel_ms": main_ms,
-                        "kernel": "k_deposit_waves<FP64>"}
-        else:
-            log("[bench] roofline skipped: tests/golden/c2_flops.json is missing")
-
         cpu = None
         if not args.no_cpu_baseline:
             try:
-                cpu = cpu_baseline(lat, tree.leaves()["order"])
-                log(f"[bench] cpu baseline: {cpu['value']:.3e} tets/s ({cpu['kind']}, {cpu['cores']} cores)")
+                cpu = cpu_baseline(lat, DIMS)
+                log(f"[bench] cpu baseline: {cpu['value']:.3e} tets/s ({cpu['kind']}, {cpu['cores']} cores; one core "
+                    f"{cpu.get('one_core_value', float('nan')):.3e})")
             except Exception as exc:
                 log(f"[bench] cpu baseline skipped: {exc}")
         tree.close()
 
+    field = f"rms {wl['rms']:g} spacings" + (f" + {wl['infall']} infall centres" if wl["infall"] else "")
     out = {
         "metric": "tets/s exact deposit; p50/p99 ROI latency vs plain GPU at \u22641e-6 mass err",
         "value": value,
@@ -702,11 +808,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded k^-2 Gaussian displacement field, rms 3 spacings)",
-        "config": {"workload": "adaptive exact deposit, 128^3-particle lattice (12,582,912 Kuhn tets), octree depth<=6, "
-                               "split >50000, 16^3-cell leaves, batched persistent FP64 form; a step = per-leaf tet "
-                               "binning + deposit of all leaves (octree refinement reported as octree_ms)",
-                   "lattice": N, "tets": lat.num_tets, "leaves": int(leaves.size), "leaf_dims": DIMS,
+        "data": f"synthetic (seeded k^-2 Gaussian displacement field, {field})",
+        "config": {"workload": f"adaptive exact deposit, {n}^3-particle lattice ({lat.num_tets:,} Kuhn tets, {field}), "
+                               f"octree depth<=6, split >50000, {DIMS}^3-cell leaves, batched persistent FP64 form; a "
+                               f"step = per-leaf tet binning + deposit of all leaves (octree refinement reported as "
+                               f"octree_ms; {DIMS_FINE}^3-cell leaves in fine_leaves)",
+                   "lattice": n, "tets": lat.num_tets, "leaves": int(leaves.size), "leaf_dims": DIMS,
+                   "binned_pairs": int(counts.sum()), "leaf_tets_min_median_max": [int(nz.min()), int(np.median(nz)), int(nz.max())],
                    "octree_ms": t_oct * 1e3, "binning_ms": t_bin * 1e3, "mass_drift": drift,
                    "launch_shape": {"tiles_per_pop": shape["chunk"], "workgroups_per_cu": shape["blocks_per_cu"]}},
         "roofline": roofline,

@@ -208,8 +208,10 @@ adeco_status adeco_write_leaves(adeco_octree tree, const char* output_path, doub
  * launch (AdDeposit2Grid.h:72-80). */
 adeco_status adeco_set_launch_shape(int chunk, int blocks_per_cu);
 adeco_status adeco_get_launch_shape(int* chunk, int* blocks_per_cu);
-/* Times the batched leaf deposit over chunk in {1,2,4,8,16,32} x blocks_per_cu in {1,2,3,4}
- * (minimum kernel time of reps runs each), keeps the fastest as the launch shape and returns it. */
+/* Times the batched leaf deposit over chunk in {1,2,4,8,16,32} x blocks_per_cu in 1..B, B the
+ * workgroups of the kernel that one CU holds at once (hipOccupancyMaxActiveBlocksPerMultiprocessor,
+ * at most 4), keeps the fastest (minimum kernel time of reps runs each) as the launch shape and
+ * returns it.  Launches never place more workgroups per CU than B, whatever the shape says. */
 adeco_status adeco_autotune_batched(adeco_octree tree, int dims, int decomposition, double tet_mass, int precision,
                                     int reps, int* best_chunk, int* best_blocks_per_cu, double* best_ms);
 /* hipGraph form of the batched leaf deposit (no reference counterpart: the reference launches per

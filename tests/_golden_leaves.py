@@ -1,11 +1,13 @@
-"""The reference's own leaf grids of the 128^3 adaptive workload (checker side).
+"""The reference's own leaf grids of the 128^3 and 256^3 adaptive workloads (checker side).
 
 tests/golden/ref_leaves_c2.npz holds, for 12 leaves of the 128^3 lattice (rms 3 spacings, seed 2;
 octree depth <= 6, split above 50,000 estimated tets; 16^3-cell leaves), the sha256 of the float
 grid that the compiled reference deposits for the leaf (its leaf gather and
 DepositWrapper::deposit_tets, AdDynamicLoadBalancing.h:983-1323), the leaf's float box and the
-grid total.  The float-faithful oracle rebuilds those grids from each leaf's meta-patches, in the
-reference's gather order, bit for bit.
+grid total.  tests/golden/ref_leaves_c3_16.npz and ref_leaves_c3_32.npz hold the same for the 6
+heaviest leaves and 4 seeded others of the 256^3 lattice with 64 infall centres (rms 4 spacings,
+seed 3), at 16^3 and 32^3 cells, plus 64 sampled cell values per grid.  The float-faithful oracle
+rebuilds those grids from each leaf's meta-patches, in the reference's gather order, bit for bit.
 """
 
 import numpy as np
@@ -15,9 +17,18 @@ from tests import _oracle as O
 from tests._octree_io import oracle_octree
 
 
-def reference_leaf_grids(g):
+# lattice of each fixture: n, rms, seed, infall centres
+CASES = {"c2": (128, 3.0, 2, 0), "c3": (256, 4.0, 3, 64)}
+
+
+def golden_lattice(case):
+    n, rms, seed, centres = CASES[case]
+    return make_lattice(n, rms, seed=seed, infall_centres=centres)
+
+
+def reference_leaf_grids(g, case="c2", lat=None):
     """Returns (lattice, oracle octree, [float grid per fixture leaf])."""
-    lat = make_lattice(128, 3.0, seed=2)
+    lat = golden_lattice(case) if lat is None else lat
     d = oracle_octree(lat, 50000, 6)
     dims = int(g["dims"])
     out = []

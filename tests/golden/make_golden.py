@@ -9,6 +9,10 @@ adeco_amd.lattice, whose output hash is stored so generator drift is detected.
   ref_octree_<n>.npz   generate_tree node tables, counts, leaf lists, meta-cell AABBs
   ref_leaves_c2.npz    per-leaf float grids (sha256 + write_patch mass) for leaves of the
                        128^3 configuration, deposited as one worker thread does
+  ref_leaves_c3_<d>.npz the same for the heaviest leaves (and a seeded few others) of the 256^3
+                       lattice with 64 infall centres, at 16^3 and 32^3 cells per leaf
+
+Run from the repository root:  python tests/golden/make_golden.py [c3]
 """
 
 from __future__ import annotations
@@ -78,12 +82,14 @@ def octree_fixture(n, rms, seed, max_tets, max_level):
     return lat, d
 
 
-def leaves_fixture(lat, d, dims=16, count=12):
+def leaves_fixture(lat, d, dims=16, count=12, pick=None, name="ref_leaves_c2.npz"):
     r = O.load_ref()
     order = d["order"]
-    pick = list(order[:4]) + list(order[-2:]) + list(np.random.default_rng(1).choice(order[4:-2], count - 6, replace=False))
+    if pick is None:
+        pick = list(order[:4]) + list(order[-2:]) + list(np.random.default_rng(1).choice(order[4:-2], count - 6,
+                                                                                        replace=False))
     rows = {"node": [], "bbox": [], "grid_sha": [], "mass": [], "density_sha": [], "grid_total": [], "seconds": [],
-            "n_meta": []}
+            "n_meta": [], "sample_idx": [], "sample_val": []}
     grid = np.zeros(dims ** 3, np.float32)
     mass = np.zeros(1)
     bbox = np.zeros(6, np.float32)
@@ -107,18 +113,42 @@ def leaves_fixture(lat, d, dims=16, count=12):
             rows["grid_total"].append(float(grid.sum(dtype=np.float64)))
             leaf = int(np.flatnonzero(d["dfs"] == node)[0])
             rows["n_meta"].append(int(d["mc_offsets"][leaf + 1] - d["mc_offsets"][leaf]))
+            nz = np.flatnonzero(grid)
+            idx = np.unique(np.concatenate([nz[:: max(1, nz.size // 48)][:48], np.argsort(-grid)[:16]]))[:64]
+            idx = np.pad(idx, (0, 64 - idx.size), mode="edge")
+            rows["sample_idx"].append(idx)
+            rows["sample_val"].append(grid[idx])
             print(f"leaf node {node}: level {lvl} mass {mass[0]:.4f} meta {rows['n_meta'][-1]} "
                   f"{rows['seconds'][-1]:.2f}s", flush=True)
-    np.savez_compressed(os.path.join(HERE, "ref_leaves_c2.npz"), dims=dims, lattice_sha=lattice_sha(lat),
+    np.savez_compressed(os.path.join(HERE, name), dims=dims, lattice_sha=lattice_sha(lat),
                         node=np.array(rows["node"]), bbox=np.array(rows["bbox"]), grid_sha=np.array(rows["grid_sha"]),
                         mass=np.array(rows["mass"]), density_sha=np.array(rows["density_sha"]),
                         grid_total=np.array(rows["grid_total"]), seconds=np.array(rows["seconds"]),
-                        n_meta=np.array(rows["n_meta"]))
+                        n_meta=np.array(rows["n_meta"]), sample_idx=np.array(rows["sample_idx"]),
+                        sample_val=np.array(rows["sample_val"], np.float32))
+
+
+C3_CASE = (256, 4.0, 3, 64, 50000, 6)  # n, rms, seed, infall centres, max tets per node, max level
+
+
+def leaves_fixture_c3():
+    """The 256^3 infall lattice: the reference octree, then the 6 leaves it processes first (the
+    heaviest by its estimate) and 4 seeded others, at 16^3 and at 32^3 cells per leaf."""
+    n, rms, seed, centres, max_tets, max_level = C3_CASE
+    lat = make_lattice(n, rms, seed=seed, infall_centres=centres)
+    d = ref_octree(lat, max_tets, max_level)
+    order = d["order"]
+    pick = list(order[:6]) + list(np.random.default_rng(3).choice(order[6:], 4, replace=False))
+    for dims in (16, 32):
+        leaves_fixture(lat, d, dims=dims, pick=pick, name=f"ref_leaves_c3_{dims}.npz")
 
 
 def main():
     if O.load_ref() is None:
         sys.exit("oracle/_ref/libadeco_ref.so is not built (run oracle/build_ref.sh)")
+    if sys.argv[1:] == ["c3"]:
+        leaves_fixture_c3()
+        return
     deposit_fixture()
     for case in OCTREE_CASES:
         lat, d = octree_fixture(*case)

@@ -1,0 +1,102 @@
+"""GPU leaf deposit on lattices with spherical infall (the heavy-tailed batched workload's field:
+k^-2 displacement at rms 4 spacings plus seeded infall centres that pull tets into caustics and
+invert some), at 16^3 and 32^3 cells per leaf.
+
+* A 56^3 infall lattice with a deep octree (heavy-tailed leaves): every leaf grid of the plain
+  per-leaf launches and of the batched persistent launch, for the heaviest leaves and a spread of
+  others, against the oracle's deposit of the same leaf in extended precision, per cell:
+  |gpu - oracle| <= 1e-6 * max(|oracle|, 1e-8 * tet_mass); total mass within 1e-9 of the lattice
+  mass inside the root.
+* The 256^3 lattice with 64 infall centres (the workload itself), in the reference's geometry
+  (float leaf boxes, 1/dx scaling, its 6-tet decomposition, tet mass 1): the 6 heaviest leaves and
+  4 seeded others, whose float grids the compiled reference produced (tests/golden/ref_leaves_c3_*
+  .npz, rebuilt bit for bit by the float-faithful oracle on the CPU), against the FP64 oracle's
+  deposit of each leaf's meta-patches to 1e-6 per cell, and against the reference's float grids
+  to the reference's own float error.  On these leaves the reference differs from the FP64 oracle
+  by up to 2.9e-3 (16^3) and 1.4e-2 (32^3) per cell relative, with a floor of 1e-3 of a tet mass,
+  and by up to 1.1e-6 (16^3) and 7.8e-6 (32^3) on leaf totals; the bars are 5e-3 / 2e-2 per cell
+  and 2e-6 / 1.5e-5 on totals.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+import adeco_amd as A
+from adeco_amd.lattice import make_lattice, meta_patches
+from tests import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-6
+TET_MASS = 1.0 / 6.0
+
+
+@pytest.mark.parametrize("dims", [16, 32])
+def test_infall_leaves_match_oracle(dims):
+    lat = make_lattice(56, 4.0, seed=3, infall_centres=24)
+    print(f"[infall] 56^3 lattice, {dims}^3-cell leaves", flush=True)
+    with O.cell_scaling(O.EXACT), A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 3000, 6) as tree:
+        nl = tree.num_leaves
+        counts = tree.bin(A.DECOMP_KUHN)
+        nz = counts[counts > 0]
+        # a heavy tail: the heaviest leaf holds many times the median's tets
+        assert nl > 100 and nz.max() > 4 * np.median(nz), (nl, nz.max(), np.median(nz))
+        levels = tree.nodes()["level"][tree.leaves()["dfs"]]
+        assert np.unique(levels).size >= 2
+        pick = np.unique(np.concatenate([np.argsort(-counts)[:4], np.linspace(0, nl - 1, 5).astype(int)]))
+        boxes = tree.leaf_boxes_f64(exact=True)
+        want = []
+        for l in pick:
+            ids = tree.leaf_tets(int(l))
+            want.append(O.deposit_lattice(lat, boxes[l], dims, decomp=A.DECOMP_KUHN, tet_mass=TET_MASS, tet_ids=ids,
+                                          precision=3, nthreads=8))
+        want = np.array(want)
+        inside = O.deposit_lattice(lat, tree.root_box_f64(), 1, decomp=A.DECOMP_KUHN, tet_mass=TET_MASS).sum()
+        for name, mode in (("plain", A.MODE_PLAIN), ("batched", A.MODE_BATCHED)):
+            tree.deposit(dims, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=mode, precision=A.PREC_FP64)
+            g = tree.leaf_grids()
+            err = O.max_rel_err(g[pick], want, tet_mass=TET_MASS)
+            drift = abs(g.sum() - inside) / inside
+            print(f"  {name}: max rel err {err:.2e}, mass drift {drift:.2e}", flush=True)
+            assert err <= REL, name
+            assert drift <= 1e-9, name
+
+
+# per-cell and per-leaf-total bars against the reference's float grids, by cells per leaf
+FLOAT_BARS = {16: (5e-3, 2e-6), 32: (2e-2, 1.5e-5)}
+
+
+@pytest.mark.parametrize("dims", [16, 32])
+def test_infall_leaf_grids_match_reference_leaf_grids(dims):
+    from tests._golden_leaves import golden_lattice, reference_leaf_grids
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"ref_leaves_c3_{dims}.npz"), allow_pickle=False)
+    assert int(g["dims"]) == dims
+    lat = golden_lattice("c3")
+    _, d, ref = reference_leaf_grids(g, "c3", lat=lat)
+    cell_bar, total_bar = FLOAT_BARS[dims]
+    f64 = []
+    with O.cell_scaling(O.REFERENCE):
+        for node, bbox in zip(g["node"], g["bbox"]):
+            leaf = int(np.flatnonzero(d["dfs"] == node)[0])
+            ids = d["mc_ids"][d["mc_offsets"][leaf]:d["mc_offsets"][leaf + 1]]
+            f64.append(O.deposit_patches_f64(meta_patches(lat, ids), bbox, dims))
+        with A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 50000, 6) as tree:
+            tree.bin(A.DECOMP_REF_EFFECTIVE)
+            dfs = tree.leaves()["dfs"]
+            boxes = tree.leaf_boxes_f64(exact=False)
+            leaves = [int(np.flatnonzero(dfs == node)[0]) for node in g["node"]]
+            for l, bbox in zip(leaves, g["bbox"]):
+                assert np.array_equal(boxes[l].astype(np.float32), bbox)
+            forms = (("plain", A.MODE_PLAIN), ("batched", A.MODE_BATCHED)) if dims == 16 else (("batched", A.MODE_BATCHED),)
+            for name, mode in forms:
+                print(f"[reference infall leaves] {dims}^3 {name}", flush=True)
+                tree.deposit(dims, decomposition=A.DECOMP_REF_EFFECTIVE, tet_mass=1.0, mode=mode)
+                grids = tree.leaf_grids()
+                for i, l in enumerate(leaves):
+                    ours = grids[l]
+                    assert O.max_rel_err(ours, f64[i], tet_mass=1.0) <= REL, (name, int(g["node"][i]))
+                    r = ref[i].astype(np.float64)
+                    assert abs(ours.sum() - r.sum()) <= total_bar * r.sum(), (name, int(g["node"][i]))
+                    assert np.max(np.abs(ours - r) / np.maximum(r, 1e-3)) <= cell_bar, (name, int(g["node"][i]))

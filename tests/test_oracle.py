@@ -272,3 +272,20 @@ def test_float_faithful_oracle_matches_reference_leaf_grids():
     for i, grid in enumerate(grids):
         assert hashlib.sha256(grid.tobytes()).hexdigest() == str(g["grid_sha"][i]), int(g["node"][i])
         assert float(grid.sum(dtype=np.float64)) == float(g["grid_total"][i])
+
+
+def test_float_faithful_oracle_matches_reference_infall_leaf_grids():
+    """Golden leaf grids of the compiled reference on the 256^3 lattice with 64 infall centres (its
+    6 heaviest leaves and 4 seeded others, 16^3 and 32^3 cells): the float-faithful oracle rebuilds
+    every one bit for bit, and the sampled cells hold the reference's values."""
+    from tests._golden_leaves import golden_lattice
+    lat = golden_lattice("c3")
+    for dims in (16, 32):
+        g = _golden(f"ref_leaves_c3_{dims}.npz")
+        assert int(g["dims"]) == dims
+        assert hashlib.sha256(np.concatenate([lat.x, lat.y, lat.z]).tobytes()).hexdigest() == str(g["lattice_sha"])
+        _, _, grids = reference_leaf_grids(g, "c3", lat=lat)
+        for i, grid in enumerate(grids):
+            assert np.array_equal(grid[g["sample_idx"][i]], g["sample_val"][i]), (dims, int(g["node"][i]))
+            assert hashlib.sha256(grid.tobytes()).hexdigest() == str(g["grid_sha"][i]), (dims, int(g["node"][i]))
+            assert float(grid.sum(dtype=np.float64)) == float(g["grid_total"][i])
