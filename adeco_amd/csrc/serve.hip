@@ -12,10 +12,14 @@
 //     the tets whose float AABB strictly overlaps the box;
 //   * steps: every step is one launch of the batched deposit kernel over (request, tile) entries of
 //     all admitted requests, each request taking an equal share of the step's tiles, so a request
-//     admitted while others run joins the next step and its tiles share the device with theirs;
+//     admitted while others run joins the next step and its tiles share the device with theirs.
+//     Steps are queued without waiting for the previous one: the step entries are double-buffered
+//     (a buffer is refilled once its upload of two steps ago has run), and admission runs on a
+//     stream of its own, so selecting and setting up new requests overlaps the running step;
 //   * completion: after a request's last tile its FP64 masses are turned into FP32 density
-//     (mass / cell volume, as AdOctreeGridWriter::write_patch) and a copier thread moves them to
-//     the caller's buffer, then frees the slot.
+//     (mass / cell volume, as AdOctreeGridWriter::write_patch) and an event is recorded on the
+//     step stream; a copier thread waits for that event, moves the density to the caller's buffer
+//     and frees the slot.
 // adeco_roi_deposit_plain is the serial baseline: the same selection, then the plain
 // one-tet-per-thread kernel, one request at a time.
 #include <hip/hip_runtime.h>
@@ -49,6 +53,7 @@ static_assert(sizeof(adeco_serve_config) == 40 && sizeof(adeco_request_info) == 
 
 constexpr int kMeta = 7;  // lattice cubes per meta-cell edge (AdTypeDefinitions.h:732-734)
 constexpr int kMaxSlots = 64;
+constexpr int64_t kMaxStepTiles = int64_t(1) << 22;  // two pinned entry buffers of 8 bytes per tile
 
 double now_s() {
     using namespace std::chrono;
@@ -295,13 +300,20 @@ struct adeco_server_s {
     adeco_serve_config cfg;
     PackedCorners ct;
     Selector sel;
-    hipStream_t stream = nullptr, copy_stream = nullptr;
+    // steps run on `stream`, request setup on `admit_stream`, downloads on `copy_stream`
+    hipStream_t stream = nullptr, admit_stream = nullptr, copy_stream = nullptr;
     std::unique_ptr<Slot[]> slot_store;
     int n_slots = 0;
     Slot* slots = nullptr;
     TileSource h_sources[kMaxSlots];
-    DevBuf d_sources, d_entries, d_ctr;
-    int2* h_entries = nullptr;  // pinned
+    hipEvent_t slot_done[kMaxSlots] = {};  // recorded after a request's density kernel
+    DevBuf d_sources, d_ctr;
+    // double-buffered step entries: pinned host side, device side, and an event recorded after
+    // each buffer's upload
+    int2* h_entries[2] = {nullptr, nullptr};
+    DevBuf d_entries[2];
+    hipEvent_t entries_uploaded[2] = {nullptr, nullptr};
+    bool entries_in_flight[2] = {false, false};
     std::thread worker, copier;
     std::mutex mu;
     std::condition_variable cv_work, cv_done, cv_copy;
@@ -309,6 +321,7 @@ struct adeco_server_s {
     std::vector<Req*> active;
     std::map<uint64_t, Req*> reqs;
     uint64_t next_id = 1;
+    uint64_t n_released = 0;  // slots freed so far (a request waiting for memory waits for this to move)
     bool stop = false;
     std::string worker_error;
     // statistics
@@ -331,6 +344,27 @@ struct adeco_server_s {
     adeco_status admit(Req* r, int si);
     void run_worker();
     void run_copier();
+    // streams, events, pinned and device buffers of the server itself (not the slots')
+    void release_device_objects() {
+        for (int b = 0; b < 2; ++b) {
+            if (h_entries[b]) (void)hipHostFree(h_entries[b]);
+            if (entries_uploaded[b]) (void)hipEventDestroy(entries_uploaded[b]);
+            h_entries[b] = nullptr;
+            entries_uploaded[b] = nullptr;
+            d_entries[b].free_all();
+        }
+        for (int i = 0; i < kMaxSlots; ++i)
+            if (slot_done[i]) {
+                (void)hipEventDestroy(slot_done[i]);
+                slot_done[i] = nullptr;
+            }
+        d_sources.free_all();
+        d_ctr.free_all();
+        if (stream) (void)hipStreamDestroy(stream);
+        if (admit_stream) (void)hipStreamDestroy(admit_stream);
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
+        stream = admit_stream = copy_stream = nullptr;
+    }
 };
 
 adeco_status adeco_server_s::admit(Req* r, int si) {
@@ -345,7 +379,7 @@ adeco_status adeco_server_s::admit(Req* r, int si) {
     auto grow = [](const DevBuf& b, size_t need, size_t floor) { return need > b.cap ? std::max(need, floor) : size_t(0); };
     size_t fixed = grow(s.grid, grid_b, min_b) + grow(s.out32, out_b, min_b) + grow(s.desc, 512, min_b) + grow(s.tiles, 16, min_b);
     auto fits = [&](size_t more) { return !cfg.memory_budget || capacity_in_use() + fixed + more <= cfg.memory_budget; };
-    adeco_status st = sel.select(stream, lv, ct, rf, s.pairs, r->n_tets,
+    adeco_status st = sel.select(admit_stream, lv, ct, rf, s.pairs, r->n_tets,
                                  [&](size_t pairs_b) { return fits(grow(s.pairs, pairs_b, min_b)); });
     if (st) return st;
     const size_t work_b = candidate_space_bytes(r->n_tets) + 256;
@@ -358,37 +392,45 @@ adeco_status adeco_server_s::admit(Req* r, int si) {
     r->goff[1] = r->n_tets;
     GridDesc* d_gd = s.desc.as<GridDesc>();
     int64_t* d_goff = reinterpret_cast<int64_t*>(s.desc.as<char>() + 256);
-    hipError_t e = hipMemcpyAsync(d_gd, &r->gd, sizeof(GridDesc), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_goff, r->goff, sizeof(r->goff), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = launch_fill_f64(s.grid.as<double>(), cells, 0.0, stream);
+    hipError_t e = hipMemcpyAsync(d_gd, &r->gd, sizeof(GridDesc), hipMemcpyHostToDevice, admit_stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_goff, r->goff, sizeof(r->goff), hipMemcpyHostToDevice, admit_stream);
+    if (e == hipSuccess) e = launch_fill_f64(s.grid.as<double>(), cells, 0.0, admit_stream);
     if (e != hipSuccess) return hip_fail(e, "request setup");
     r->n_cand = 0;
     r->n_tiles = 0;
     if (r->n_tets > 0) {
         CandidateSpace cs;
-        if ((st = build_candidate_space(stream, lv, ct, s.pairs.as<int32_t>(), r->n_tets, d_goff, 1, d_gd, cfg.tet_mass,
+        if ((st = build_candidate_space(admit_stream, lv, ct, s.pairs.as<int32_t>(), r->n_tets, d_goff, 1, d_gd, cfg.tet_mass,
                                         cfg.precision, s.work.ptr, cs)))
             return st;
         if ((st = s.tiles.reserve(tile_map_bytes(cs.n_cand)))) return st;
-        if ((st = build_tile_map(stream, cs, s.tiles.as<int32_t>()))) return st;
+        if ((st = build_tile_map(admit_stream, cs, s.tiles.as<int32_t>()))) return st;
         r->n_cand = cs.n_cand;
         r->n_tiles = cs.n_tiles;
         h_sources[si] = TileSource{cs.pd, cs.cand_off, cs.tile_first, cs.n_tiles, cs.n_cand, d_gd, s.grid.as<double>(),
                                    cfg.tet_mass};
-        e = hipMemcpyAsync(d_sources.as<TileSource>() + si, &h_sources[si], sizeof(TileSource), hipMemcpyHostToDevice, stream);
+        e = hipMemcpyAsync(d_sources.as<TileSource>() + si, &h_sources[si], sizeof(TileSource), hipMemcpyHostToDevice,
+                           admit_stream);
         if (e != hipSuccess) return hip_fail(e, "request setup");
     }
     r->next_tile = 0;
-    return ADECO_OK;
+    // the request joins the next step launch on the step stream: its setup must have run
+    e = hipStreamSynchronize(admit_stream);
+    return e == hipSuccess ? ADECO_OK : hip_fail(e, "request setup");
 }
 
 void adeco_server_s::run_worker() {
     (void)hipSetDevice(L->device);
     std::unique_lock<std::mutex> lk(mu);
+    int buf = 0;  // step-entries buffer of the next step
     for (;;) {
         cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
         if (stop && pending.empty() && active.empty()) break;
-        // admission: in arrival order, while a slot is free and the memory budget allows
+        // admission: in arrival order, while a slot is free and the memory budget allows.  A request
+        // that does not fit waits while another slot holds a request (running, or being copied out:
+        // its slot keeps its memory until the copier frees it); idle slots' buffers are released
+        // before it waits or gives up, and it fails only when nothing else holds memory.
+        uint64_t seen_release = n_released;
         while (!pending.empty()) {
             int si = -1;
             for (int i = 0; i < n_slots; ++i)
@@ -396,23 +438,39 @@ void adeco_server_s::run_worker() {
                     si = i;
                     break;
                 }
-            if (si < 0) break;
+            if (si < 0) {
+                seen_release = n_released;  // every slot is taken: wait for one to be freed
+                break;
+            }
             Req* r = pending.front();
             slots[size_t(si)].busy = true;
             lk.unlock();
             adeco_status st = admit(r, si);
-            if (st == ADECO_ENOMEM && cfg.memory_budget) {
-                // release idle slots' buffers and retry once; then wait for running requests
+            bool must_wait = false;
+            while (st == ADECO_ENOMEM && cfg.memory_budget) {
                 lk.lock();
-                for (int i = 0; i < n_slots; ++i)
-                    if (!slots[i].busy) slots[i].release();
+                bool others_busy = false, freed = false;
+                for (int i = 0; i < n_slots; ++i) {
+                    if (i == si) continue;
+                    if (slots[i].busy) {
+                        others_busy = true;
+                    } else if (slots[i].capacity()) {
+                        slots[i].release();
+                        freed = true;
+                    }
+                }
+                if (!freed && others_busy) {
+                    must_wait = true;
+                    seen_release = n_released;  // read under the lock that decided to wait
+                }
                 lk.unlock();
+                if (!freed) break;
                 st = admit(r, si);
             }
             lk.lock();
-            if (st == ADECO_ENOMEM && cfg.memory_budget && !(active.empty() && to_copy.empty())) {
+            if (must_wait) {
                 slots[size_t(si)].busy = false;
-                break;  // admitted once running requests release memory
+                break;
             }
             pending.pop_front();
             r->slot = si;
@@ -422,6 +480,7 @@ void adeco_server_s::run_worker() {
                 r->done = true;
                 r->t_done = now_s();
                 slots[size_t(si)].busy = false;
+                ++n_released;
                 ++n_done;
                 cv_done.notify_all();
                 continue;
@@ -429,13 +488,22 @@ void adeco_server_s::run_worker() {
             active.push_back(r);
         }
         if (active.empty()) {
-            if (!pending.empty()) cv_work.wait(lk);  // a copier release will wake us
+            // nothing to step: the head request waits until a slot is freed (the slots it waits
+            // for are being copied out; a copy that ended since seen_release was read counts)
+            if (!pending.empty()) cv_work.wait(lk, [&] { return n_released != seen_release; });
             continue;
         }
         // one step: an equal share of the step's tiles to every admitted request
         std::vector<Req*> run = active;
         lk.unlock();
         const double t0 = now_s();
+        hipError_t e = hipSuccess;
+        // this buffer's previous upload (two steps ago) must have run before it is refilled
+        if (entries_in_flight[buf]) {
+            e = hipEventSynchronize(entries_uploaded[buf]);
+            entries_in_flight[buf] = false;
+        }
+        int2* he = h_entries[buf];
         const int64_t cap = cfg.step_tiles;
         int n = 0;
         int64_t left = cap;
@@ -448,23 +516,25 @@ void adeco_server_s::run_worker() {
                 const int64_t rem = r->n_tiles - r->next_tile;
                 if (rem <= 0 || left <= 0) continue;
                 const int64_t take = std::min(std::min(rem, quota), left);
-                for (int64_t t = 0; t < take; ++t) h_entries[n++] = make_int2(r->slot, int(r->next_tile + t));
+                for (int64_t t = 0; t < take; ++t) he[n++] = make_int2(r->slot, int(r->next_tile + t));
                 r->next_tile += take;
                 left -= take;
                 if (r->next_tile < r->n_tiles) ++open;
             }
         }
-        hipError_t e = hipSuccess;
         unsigned* ctr = d_ctr.as<unsigned>();
         unsigned long long* counters = reinterpret_cast<unsigned long long*>(d_ctr.as<char>() + 16);
-        if (n > 0) {
-            e = hipMemcpyAsync(d_entries.ptr, h_entries, size_t(n) * sizeof(int2), hipMemcpyHostToDevice, stream);
+        if (n > 0 && e == hipSuccess) {
+            e = hipMemcpyAsync(d_entries[buf].ptr, he, size_t(n) * sizeof(int2), hipMemcpyHostToDevice, stream);
+            if (e == hipSuccess) e = hipEventRecord(entries_uploaded[buf], stream);
+            if (e == hipSuccess) entries_in_flight[buf] = true;
             if (e == hipSuccess) e = launch_zero_u32(ctr, 16, stream);
             if (e == hipSuccess)
-                e = launch_deposit_step(stream, cfg.precision, d_sources.as<TileSource>(), d_entries.as<int2>(), n, ctr,
+                e = launch_deposit_step(stream, cfg.precision, d_sources.as<TileSource>(), d_entries[buf].as<int2>(), n, ctr,
                                         counters, L->device);
         }
-        // requests whose last tile ran in this step: density in FP32 for the copier
+        buf ^= 1;
+        // requests whose last tile is in this step: density in FP32, then an event for the copier
         std::vector<Req*> finished;
         for (Req* r : run)
             if (r->next_tile >= r->n_tiles) {
@@ -476,9 +546,9 @@ void adeco_server_s::run_worker() {
                                        stream, s.grid.as<double>(), cells, 1.0 / g.cell_vol, s.out32.as<float>());
                     e = hipGetLastError();
                 }
+                if (e == hipSuccess) e = hipEventRecord(slot_done[r->slot], stream);
                 finished.push_back(r);
             }
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);  // h_entries is reused by the next step
         const double t1 = now_s();
         lk.lock();
         busy_s += t1 - t0;
@@ -491,6 +561,7 @@ void adeco_server_s::run_worker() {
                 r->done = true;
                 r->t_done = now_s();
                 slots[size_t(r->slot)].busy = false;
+                ++n_released;
                 ++n_done;
             }
             active.clear();
@@ -503,6 +574,9 @@ void adeco_server_s::run_worker() {
         }
         if (!finished.empty()) cv_copy.notify_one();
     }
+    lk.unlock();
+    (void)hipStreamSynchronize(stream);
+    lk.lock();
     cv_copy.notify_all();
 }
 
@@ -517,13 +591,16 @@ void adeco_server_s::run_copier() {
         Slot& s = slots[size_t(r->slot)];
         const size_t bytes = size_t(r->dims) * r->dims * r->dims * sizeof(float);
         lk.unlock();
-        hipError_t e = hipMemcpyAsync(r->host_out, s.out32.ptr, bytes, hipMemcpyDeviceToHost, copy_stream);
+        // the request's last step and density kernel, then the download
+        hipError_t e = hipEventSynchronize(slot_done[r->slot]);
+        if (e == hipSuccess) e = hipMemcpyAsync(r->host_out, s.out32.ptr, bytes, hipMemcpyDeviceToHost, copy_stream);
         if (e == hipSuccess) e = hipStreamSynchronize(copy_stream);
         lk.lock();
         if (e != hipSuccess) r->status = hip_fail(e, "request download");
         r->t_done = now_s();
         r->done = true;
         s.busy = false;
+        ++n_released;
         ++n_done;
         tets_done += r->n_tets;
         cv_done.notify_all();
@@ -540,7 +617,8 @@ adeco_status adeco_serve_create(adeco_lattice L, const adeco_serve_config* cfg, 
     if (cfg->precision < 0 || cfg->precision > 2) return fail(ADECO_EINVAL, "serve_create(): unknown precision");
     if (cfg->max_inflight < 1 || cfg->max_inflight > kMaxSlots)
         return fail(ADECO_EINVAL, "serve_create(): max_inflight must be in 1..64");
-    if (cfg->step_tiles < 0) return fail(ADECO_EINVAL, "serve_create(): negative step_tiles");
+    if (cfg->step_tiles < 0 || cfg->step_tiles > kMaxStepTiles)
+        return fail(ADECO_EINVAL, "serve_create(): step_tiles must be in 0..4194304");
     if (!(cfg->tet_mass > 0.0)) return fail(ADECO_EINVAL, "serve_create(): tet_mass must be positive");
     auto* S = new adeco_server_s();
     S->L = L;
@@ -552,17 +630,21 @@ adeco_status adeco_serve_create(adeco_lattice L, const adeco_serve_config* cfg, 
     if (!st) {
         e = hipSetDevice(L->device);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->admit_stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_entries), size_t(S->cfg.step_tiles) * sizeof(int2));
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+            e = hipHostMalloc(reinterpret_cast<void**>(&S->h_entries[b]), size_t(S->cfg.step_tiles) * sizeof(int2));
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&S->entries_uploaded[b], hipEventDisableTiming);
+        }
+        for (int i = 0; i < cfg->max_inflight && e == hipSuccess; ++i)
+            e = hipEventCreateWithFlags(&S->slot_done[i], hipEventDisableTiming);
         if (e != hipSuccess) st = hip_fail(e, "serve_create()");
     }
     if (!st) st = S->d_sources.reserve(sizeof(TileSource) * kMaxSlots);
-    if (!st) st = S->d_entries.reserve(size_t(S->cfg.step_tiles) * sizeof(int2));
+    for (int b = 0; b < 2 && !st; ++b) st = S->d_entries[b].reserve(size_t(S->cfg.step_tiles) * sizeof(int2));
     if (!st) st = S->d_ctr.reserve(64);
     if (st) {
-        if (S->h_entries) (void)hipHostFree(S->h_entries);
-        if (S->stream) (void)hipStreamDestroy(S->stream);
-        if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
+        S->release_device_objects();
         delete S;
         return st;
     }
@@ -655,14 +737,9 @@ adeco_status adeco_serve_destroy(adeco_server S) {
     if (S->copier.joinable()) S->copier.join();
     (void)hipSetDevice(S->L->device);
     for (auto& kv : S->reqs) delete kv.second;
-    if (S->h_entries) (void)hipHostFree(S->h_entries);
     S->slot_store.reset();
     S->slots = nullptr;
-    S->d_sources.free_all();
-    S->d_entries.free_all();
-    S->d_ctr.free_all();
-    if (S->stream) (void)hipStreamDestroy(S->stream);
-    if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
+    S->release_device_objects();
     delete S;
     return ADECO_OK;
 }

@@ -336,10 +336,19 @@ def serve_study(rates, count, seed=5, max_inflight=16):
                 finish = max(finish, arr[i]) + service[i]
                 lat_p.append(finish - arr[i])
             q50, q99 = percentiles(lat_p)
+            by_res = {}
+            for d in SERVE_RES:
+                sel = [i for i in range(count) if int(dims[i]) == d]
+                if sel:
+                    b50, b99 = percentiles([lat_s[i] for i in sel])
+                    s50, s99 = percentiles([lat_p[i] for i in sel])
+                    by_res[f"{d}^3"] = {"requests": len(sel), "batched_p50_s": b50, "batched_p99_s": b99,
+                                        "plain_serial_p50_s": s50, "plain_serial_p99_s": s99}
             rec = {"rate_req_s": rate, "batched_p50_s": p50, "batched_p99_s": p99,
                    "batched_tets_per_s": tets / makespan, "batched_makespan_s": makespan, "steps": int(st["steps"]),
+                   "step_tiles": int(st["tiles"]), "worker_busy_s": float(st["busy_s"]),
                    "plain_serial_p50_s": q50, "plain_serial_p99_s": q99, "plain_serial_tets_per_s": tets / finish,
-                   "plain_serial_makespan_s": finish, "batched_vs_plain_max_rel_err": err}
+                   "plain_serial_makespan_s": finish, "batched_vs_plain_max_rel_err": err, "by_resolution": by_res}
             log(f"[serve] {rate:g} req/s: batched p50 {p50 * 1e3:.1f} ms p99 {p99 * 1e3:.1f} ms "
                 f"({tets / makespan:.3e} tets/s); serial plain p50 {q50 * 1e3:.1f} ms p99 {q99 * 1e3:.1f} ms "
                 f"({tets / finish:.3e} tets/s); max rel diff {err:.2e}")

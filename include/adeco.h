@@ -315,7 +315,8 @@ typedef struct {
     double tet_mass;      /* mass of one tet (> 0) */
     int max_inflight;     /* requests resident on the device at once, 1..64 */
     int pad_;
-    int64_t step_tiles;   /* tiles of 64 (tet, candidate) pairs per step launch (0: 131072) */
+    int64_t step_tiles;   /* tiles of 64 (tet, candidate) pairs per step launch (0: 131072; at most
+                           * 4194304: two pinned entry buffers of 8 bytes per tile) */
     uint64_t memory_budget; /* device bytes for request buffers (0: no limit but the device's) */
 } adeco_serve_config;
 typedef struct {
@@ -338,7 +339,9 @@ typedef struct {
 } adeco_serve_stats;
 /* Starts a server (a worker and a copier thread) on the lattice's device.  Requests are admitted
  * in arrival order while a slot is free and the memory budget allows; each step launch deposits an
- * equal share of its tiles from every admitted request (continuous batching). */
+ * equal share of its tiles from every admitted request (continuous batching), and steps are queued
+ * without waiting for the previous one.  A request that does not fit the budget waits while other
+ * requests hold memory and fails with ADECO_EOVERFLOW only when it cannot fit alone. */
 adeco_status adeco_serve_create(adeco_lattice lat, const adeco_serve_config* cfg, adeco_server* out);
 /* Queues a request; density_out (dims^3 floats, i fastest) must stay valid until it completes. */
 adeco_status adeco_serve_submit(adeco_server srv, const double* roi6, int dims, float* density_out,

@@ -5,7 +5,8 @@
   reference sources are present.
 * include/adeco_deposit_wrapper.hpp: a small C++ program using the reference's call pattern is
   compiled, linked against libadeco_amd.so and run.  Without a GPU it must report the error as an
-  exception (no exit, no CPU fallback); on the GPU it deposits one lattice and prints the mass.
+  exception (no exit, no CPU fallback); on the GPU it deposits one lattice, prints the mass and
+  writes the grid, which matches the oracle per cell.
 """
 
 import os
@@ -44,6 +45,9 @@ int main(int argc, char** argv) {
     double m = 0;
     for (float g : grid) m += g;
     std::printf("mass %.6f\n", m);
+    FILE* o = std::fopen(argv[9], "wb");
+    if (!o || std::fwrite(grid.data(), 4, grid.size(), o) != grid.size()) return 4;
+    std::fclose(o);
     return 0;
 }
 """
@@ -70,22 +74,31 @@ def _run_program():
         r = subprocess.run(["g++", "-std=c++17", "-O1", f"-I{ROOT}/include", src, "-o", exe, f"-L{lib_dir}",
                             "-l:libadeco_amd.so", f"-Wl,-rpath,{lib_dir}"], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
-        r = subprocess.run([exe, data, str(pat.size)] + [repr(float(v)) for v in box], capture_output=True, text=True,
-                           timeout=120)
+        gfile = os.path.join(d, "grid.bin")
+        r = subprocess.run([exe, data, str(pat.size)] + [repr(float(v)) for v in box] + [gfile], capture_output=True,
+                           text=True, timeout=120)
         assert r.returncode == 0, r.stderr
-        return r.stdout.strip(), lat
+        grid = np.fromfile(gfile, np.float32) if os.path.exists(gfile) else None
+        return r.stdout.strip(), lat, pat, np.asarray(box, np.float32), grid
 
 
 def test_cpp_program_reports_edevice_without_gpu():
     if A.device_count() > 0:
         pytest.skip("a GPU is present")
-    out, _ = _run_program()
+    out = _run_program()[0]
     assert out.startswith(f"error {A.ADECO_EDEVICE}"), out
 
 
 @pytest.mark.gpu
 def test_cpp_program_deposits_on_gpu():
-    out, lat = _run_program()
+    """The program's grid, cell by cell, against the extended-precision oracle's deposit of the same
+    meta-patches (the library's default geometry: per-axis cell scaling) to 1e-6 (floor 1e-8 of a
+    tet mass), and the total to 1e-6."""
+    from tests import _oracle as O
+    out, lat, pat, box, grid = _run_program()
     assert out.startswith("mass "), out
+    with O.cell_scaling(O.EXACT, library=False):
+        want = O.deposit_patches_ld(pat, box, 16)
+    assert O.max_rel_err(grid.astype(np.float64), want) <= 1e-6
     mass = float(out.split()[1])
-    assert abs(mass - 6 * lat.n ** 3) < 1e-2 * 6 * lat.n ** 3
+    assert abs(mass - want.sum()) <= 1e-6 * want.sum()

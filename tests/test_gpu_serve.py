@@ -106,16 +106,22 @@ def test_same_alone_and_in_a_mix(lat):
 
 
 def test_memory_budget_admission(lat):
-    """A budget that holds about one request at a time still serves all of them (admission waits
-    for frees); a request larger than the whole budget fails alone."""
+    """A budget that holds one request's buffers but not two (each 48^3 request needs about 6 MiB:
+    six buffers of at least 1 MiB) still serves every request: a request that does not fit waits
+    until the running one has been copied out and its slot freed, and is then admitted; a request
+    larger than the whole budget fails alone."""
     reqs = [([0.0, 50.0, 0.0, 50.0, 0.0, 50.0], 48), ([40.0, 90.0, 40.0, 90.0, 40.0, 90.0], 48),
             ([20.0, 70.0, 20.0, 70.0, 20.0, 70.0], 48)]
-    budget = 96 << 20
+    budget = 10 << 20
     with A.ResidentLattice.from_lattice(lat) as rl:
         with A.Server(rl, DECOMP, TET_MASS, max_inflight=4, memory_budget=budget) as srv:
             ids = [srv.submit(*r) for r in reqs]
             res = [srv.wait(i, timeout=120) for i in ids]
             assert srv.stats()["device_bytes"] <= budget
+        infos = [info for _, info in res]
+        # one request at a time: each later request was admitted only after the one before it ended
+        for prev, nxt in zip(infos, infos[1:]):
+            assert nxt["t_admit"] >= prev["t_done"], (prev, nxt)
         for (roi, dims), (got, info) in zip(reqs, res):
             want, _, cell_vol = oracle_density(lat, roi, dims)
             assert rel_err(got, want, 1e-8 * TET_MASS / cell_vol) <= 1e-6 + 1.2e-7
@@ -124,6 +130,17 @@ def test_memory_budget_admission(lat):
             with pytest.raises(A.AdRuntimeException) as ei:
                 srv.wait(rid, timeout=60)
             assert ei.value.status == A.ADECO_EOVERFLOW
+
+
+def test_step_tiles_limit(lat):
+    with A.ResidentLattice.from_lattice(lat) as rl:
+        with pytest.raises(A.AdRuntimeException) as ei:
+            A.Server(rl, DECOMP, TET_MASS, step_tiles=(1 << 22) + 1)
+        assert ei.value.status == A.ADECO_EINVAL
+        with A.Server(rl, DECOMP, TET_MASS, step_tiles=1 << 22) as srv:
+            got, info = srv.wait(srv.submit(*ROIS[0]), timeout=60)
+        want, _, cell_vol = oracle_density(lat, *ROIS[0])
+        assert rel_err(got, want, 1e-8 * TET_MASS / cell_vol) <= 1e-6 + 1.2e-7
 
 
 def test_box_outside_the_lattice_is_empty(lat):
