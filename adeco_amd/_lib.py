@@ -37,6 +37,9 @@ SIGNATURES = {
     "adeco_get_cell_scaling": (ctypes.c_int, []),
     "adeco_deposit_patches": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
                                              c_vp, ctypes.c_size_t, ctypes.c_int, c_vp, ctypes.c_int, c_vp]),
+    "adeco_deposit_patches_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, c_vp,
+                                                     ctypes.c_size_t, ctypes.c_int, c_vp, ctypes.c_int, c_vp]),
+    "adeco_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "adeco_get_grid_size": (ctypes.c_int, []),
     "adeco_get_block_size": (ctypes.c_int, []),
     "adeco_set_memory_budget": (ctypes.c_int, [ctypes.c_size_t]),

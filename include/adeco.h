@@ -99,6 +99,20 @@ adeco_status adeco_deposit_patches(int use_specific_gpu, size_t device_id, size_
                                    size_t num_patches, const float* positions, size_t positions_size,
                                    int meta_cells_per_patch, const float* bbox, int dims,
                                    float* grid_accum);
+/* adeco_deposit_patches in the plain form on one device, allocating nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Arguments and accumulation as
+ * adeco_deposit_patches with use_specific_gpu set (library-wide tet decomposition and tet mass;
+ * the kernel is always the plain FP64 one).  Device memory beyond the caller's buffers comes from a
+ * fixed 48 MiB array each GPU holds from library load to process end (adeco_noalloc_scratch_bytes):
+ * the FP64 grid and the staged float grid take 12 * dims^3 bytes of it, the meta-patches stream
+ * through the rest, so dims is at most 161 with 7^3-cell patches; larger grids are ADECO_EOVERFLOW.
+ * Calls on one device share its array and serialise on that device's lock. */
+adeco_status adeco_deposit_patches_noalloc(int device_id, size_t patch_offset, size_t num_patches,
+                                           const float* positions, size_t positions_size,
+                                           int meta_cells_per_patch, const float* bbox, int dims,
+                                           float* grid_accum);
+size_t adeco_noalloc_scratch_bytes(void);
 int adeco_get_grid_size(void);
 int adeco_get_block_size(void);
 /* Device-memory budget per call in bytes (0 = 75% of device memory, as the reference's
