@@ -213,6 +213,14 @@ class ResidentLattice:
         _check(load().adeco_lattice_bbox(self.handle, ptr(out)), "lattice_bbox()")
         return out
 
+    def update(self, x, y, z):
+        """Overwrites the resident positions with a new state of the same lattice (float32 SoA)."""
+        n = self.dims[0] * self.dims[1] * self.dims[2]
+        arrs = [np.ascontiguousarray(a, np.float32).ravel() for a in (x, y, z)]
+        if any(a.size != n for a in arrs):
+            raise AdRuntimeException(ADECO_EINVAL, "ERROR: lattice_update(): positions do not match the lattice")
+        _check(load().adeco_lattice_update(self.handle, ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2])), "lattice_update()")
+
     def positions(self):
         """The resident vertex coordinates (x, y, z), float32 SoA."""
         n = self.dims[0] * self.dims[1] * self.dims[2]
@@ -421,17 +429,27 @@ class Octree:
 
 
 class DepositGraph:
-    """The batched leaf deposit of an octree captured as a hipGraph (grid clear, work-queue reset,
-    persistent deposit launch) and replayed by ``launch()``; ``leaf_grids()`` reads the graph's own
-    FP64 leaf grids (DFS leaf order)."""
+    """The batched leaf deposit of an octree captured as a hipGraph and replayed by ``launch()``;
+    ``leaf_grids()`` reads the graph's own FP64 leaf grids (DFS leaf order).
+
+    ``step=False``: grid clear, work-queue reset and persistent deposit of the candidate space built
+    at creation.  ``step=True``: the whole step (binning of the lattice's current positions, pair
+    setup, scans, tile map, deposit) with device-side sizes, so replays follow
+    ``ResidentLattice.update``; buffers hold ``slack`` times the capture's work, beyond which a
+    replay raises ADECO_EOVERFLOW."""
 
     def __init__(self, tree: "Octree", dims: int, decomposition: int = DECOMP_REF_EFFECTIVE, tet_mass: float = 1.0,
-                 precision: int = PREC_FP64):
+                 precision: int = PREC_FP64, step: bool = False, slack: float = 1.25):
         self.tree = tree
         self.dims = int(dims)
         h = _lib.c_vp()
-        _check(load().adeco_deposit_graph_create(tree.handle, int(dims), int(decomposition), float(tet_mass),
-                                                 int(precision), _lib.ctypes.byref(h)), "deposit_graph_create()")
+        if step:
+            _check(load().adeco_deposit_graph_create_step(tree.handle, int(dims), int(decomposition), float(tet_mass),
+                                                          int(precision), float(slack), _lib.ctypes.byref(h)),
+                   "deposit_graph_create_step()")
+        else:
+            _check(load().adeco_deposit_graph_create(tree.handle, int(dims), int(decomposition), float(tet_mass),
+                                                     int(precision), _lib.ctypes.byref(h)), "deposit_graph_create()")
         self.handle = h
 
     def launch(self) -> float:

@@ -46,6 +46,7 @@ SIGNATURES = {
     "adeco_lattice_create": (ctypes.c_int, [ctypes.c_int, c_vp, c_vp, c_vp, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.POINTER(c_vp)]),
     "adeco_lattice_destroy": (ctypes.c_int, [c_vp]),
+    "adeco_lattice_update": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "adeco_lattice_bbox": (ctypes.c_int, [c_vp, c_vp]),
     "adeco_lattice_periodic_unwrap": (ctypes.c_int, [c_vp, c_vp, c_vp]),
     "adeco_lattice_positions": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
@@ -88,6 +89,8 @@ SIGNATURES = {
                                               ctypes.c_int, c_vp, c_vp, c_vp]),
     "adeco_deposit_graph_create": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                                   ctypes.POINTER(c_vp)]),
+    "adeco_deposit_graph_create_step": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                       ctypes.c_double, ctypes.POINTER(c_vp)]),
     "adeco_deposit_graph_launch": (ctypes.c_int, [c_vp]),
     "adeco_deposit_graph_grids": (ctypes.c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_vp]),
     "adeco_deposit_graph_destroy": (ctypes.c_int, [c_vp]),

@@ -173,7 +173,8 @@ __device__ __forceinline__ void pair_setup_one(LatticeView lat, PackedCorners ct
 }
 
 __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorners ct, const int32_t* __restrict__ pairs,
-                                                    int64_t n_pairs, const int64_t* __restrict__ grid_off, int n_grids,
+                                                    int64_t n_pairs, const int64_t* __restrict__ n_dev,
+                                                    const int64_t* __restrict__ grid_off, int n_grids,
                                                     const int32_t* __restrict__ pair_grid,
                                                     const GridDesc* __restrict__ grids, double tet_mass, int max_seg,
                                                     unsigned* __restrict__ cand, PairData* __restrict__ pd) {
@@ -181,7 +182,10 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
     // row writes over the banks) and stored as whole contiguous rows of the block
     constexpr int kWords = int(sizeof(PairData) / 8);
     __shared__ unsigned long long rows[256 * (kWords + 1)];
+    // the pair count: n_pairs, or the device's count capped at n_pairs (a uniform exit per block)
+    if (n_dev) n_pairs = min(*n_dev, n_pairs);
     const int64_t base = int64_t(blockIdx.x) * blockDim.x;
+    if (base >= n_pairs) return;
     const int64_t p = base + threadIdx.x;
     if (p < n_pairs) {
         PairData o;
@@ -195,20 +199,28 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
     for (int k = threadIdx.x; k < rows_here * kWords; k += 256) dst[k] = rows[(k / kWords) * (kWords + 1) + k % kWords];
 }
 
-// grid index of every pair of grid-major pairs: one workgroup per grid writes its range
-__global__ void __launch_bounds__(256) k_pair_grid(const int64_t* __restrict__ grid_off, int32_t* __restrict__ pair_grid) {
+// grid index of every pair of grid-major pairs below cap: one workgroup per grid writes its range
+__global__ void __launch_bounds__(256) k_pair_grid(const int64_t* __restrict__ grid_off, int64_t cap,
+                                                   int32_t* __restrict__ pair_grid) {
     const int g = int(blockIdx.x);
-    const int64_t end = grid_off[g + 1];
+    const int64_t end = min(grid_off[g + 1], cap);
     for (int64_t i = grid_off[g] + threadIdx.x; i < end; i += blockDim.x) pair_grid[i] = g;
 }
 
 // exclusive scan of uint32 counts into int64 offsets: block-local pass, block-sum pass, add pass
 constexpr int kScanBlock = 1024;
 
-__global__ void __launch_bounds__(256) k_scan_local(const unsigned* __restrict__ in, int64_t n, int64_t* __restrict__ out,
+// (n_dev: the count on the device, capped at n; blocks beyond it contribute zero)
+__global__ void __launch_bounds__(256) k_scan_local(const unsigned* __restrict__ in, int64_t n,
+                                                    const int64_t* __restrict__ n_dev, int64_t* __restrict__ out,
                                                     int64_t* __restrict__ block_sum) {
     __shared__ int64_t s[kScanBlock];
+    if (n_dev) n = min(*n_dev, n);
     const int64_t base = int64_t(blockIdx.x) * kScanBlock;
+    if (base >= n) {
+        if (threadIdx.x == 0) block_sum[blockIdx.x] = 0;
+        return;
+    }
     for (int i = threadIdx.x; i < kScanBlock; i += blockDim.x) s[i] = (base + i < n) ? int64_t(in[base + i]) : 0;
     __syncthreads();
     for (int off = 1; off < kScanBlock; off <<= 1) {
@@ -230,6 +242,7 @@ __global__ void __launch_bounds__(256) k_scan_local(const unsigned* __restrict__
 __global__ void __launch_bounds__(256) k_scan_sums(int64_t* __restrict__ bs, int64_t nb, int64_t* __restrict__ total) {
     __shared__ int64_t s[256];
     int64_t carry = 0;
+    // (blocks past a device-side count hold zero, so scanning all nb of them is the same)
     for (int64_t base = 0; base < nb; base += 256) {
         const int64_t i = base + threadIdx.x;
         const int64_t v = i < nb ? bs[i] : 0;
@@ -248,15 +261,22 @@ __global__ void __launch_bounds__(256) k_scan_sums(int64_t* __restrict__ bs, int
     if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ void __launch_bounds__(256) k_scan_add(int64_t* __restrict__ out, int64_t n, const int64_t* __restrict__ bs,
-                                                  const int64_t* __restrict__ total) {
+__global__ void __launch_bounds__(256) k_scan_add(int64_t* __restrict__ out, int64_t n, const int64_t* __restrict__ n_dev,
+                                                  const int64_t* __restrict__ bs, const int64_t* __restrict__ total) {
+    if (n_dev) n = min(*n_dev, n);
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) out[i] += bs[i / kScanBlock];
     if (i == n) out[n] = *total;
 }
 
+// (sizes: when given, the pair count capped at n_pairs and the tile count sizes[1] come from the
+// device; with no tiles the map is left alone)
 __global__ void __launch_bounds__(256) k_tile_map(const int64_t* __restrict__ off, int64_t n_pairs,
+                                                  const int64_t* __restrict__ n_dev, const int64_t* __restrict__ sizes,
                                                   int32_t* __restrict__ tile_first, int64_t n_tiles) {
+    if (n_dev) n_pairs = min(*n_dev, n_pairs);
+    if (sizes) n_tiles = sizes[1];
+    if (n_tiles <= 0) return;
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p > n_pairs) return;
     if (p == n_pairs) {
@@ -265,6 +285,20 @@ __global__ void __launch_bounds__(256) k_tile_map(const int64_t* __restrict__ of
     }
     const int64_t t0 = (off[p] + kWaveTile - 1) / kWaveTile, t1 = (off[p + 1] + kWaveTile - 1) / kWaveTile;
     for (int64_t t = t0; t < t1 && t < n_tiles; ++t) tile_first[t] = int32_t(p);
+}
+
+// Device-side sizes of a candidate space whose pair count only the device knows: sizes[0] = the
+// candidates, sizes[1] = the tiles; a pair count above pair_cap or a tile count above tile_cap
+// sets *overflow and leaves no tiles (the deposit then does nothing).
+__global__ void k_space_sizes(const int64_t* __restrict__ off, const int64_t* __restrict__ n_dev, int64_t pair_cap,
+                              int64_t tile_cap, int64_t* __restrict__ sizes, unsigned* __restrict__ overflow) {
+    const int64_t n = *n_dev;
+    const int64_t nc = n <= pair_cap ? off[n] : 0;
+    const int64_t nt = (nc + kWaveTile - 1) / kWaveTile;
+    const bool over = n > pair_cap || nt > tile_cap;
+    sizes[0] = over ? 0 : nc;
+    sizes[1] = over ? 0 : nt;
+    if (over) *overflow = 1u;
 }
 
 template <class R>
@@ -698,6 +732,10 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
 template <int kPrec, bool kMergeDrain>
 __global__ void __launch_bounds__(kBlock, 2) k_deposit_waves(TileSource src, int chunk, unsigned* __restrict__ queue_counter,
                                                           unsigned long long* __restrict__ counters) {
+    if (src.dev_sizes) {
+        src.n_cand = src.dev_sizes[0];
+        src.n_tiles = src.dev_sizes[1];
+    }
     wave_loop<kPrec, kMergeDrain>(OneSource{src}, nullptr, src.n_tiles, chunk, queue_counter, counters);
 }
 
@@ -757,10 +795,26 @@ size_t candidate_space_bytes(int64_t n_pairs) {
 
 size_t tile_map_bytes(int64_t n_cand) { return align256(size_t((n_cand + kWaveTile - 1) / kWaveTile + 1) * 4); }
 
-adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, const PackedCorners& ct,
-                                   const int32_t* pairs, int64_t n_pairs, const int64_t* grid_off, int n_grids,
-                                   const GridDesc* d_grids, double tet_mass, int precision, void* slab,
-                                   CandidateSpace& cs) {
+size_t scan_block_words(int64_t n) { return size_t((n + kScanBlock - 1) / kScanBlock + 1); }
+
+hipError_t launch_scan_u32(const unsigned* in, int64_t n, int64_t* out, int64_t* block_sum, int64_t* total,
+                           hipStream_t stream) {
+    const int64_t nblk = (n + kScanBlock - 1) / kScanBlock;
+    if (nblk > 0)
+        hipLaunchKernelGGL(k_scan_local, dim3(unsigned(nblk)), dim3(256), 0, stream, in, n, (const int64_t*)nullptr, out,
+                           block_sum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, stream, block_sum, nblk, total);  // (nblk 0: total 0)
+    hipLaunchKernelGGL(k_scan_add, dim3(unsigned((n + 1 + 255) / 256)), dim3(256), 0, stream, out, n,
+                       (const int64_t*)nullptr, block_sum, total);
+    return hipGetLastError();
+}
+
+// Lays out the slab of a candidate space for up to n_pairs pairs and enqueues pair setup and the
+// candidate scan; n_dev (nullable) is the pair count on the device, capped at n_pairs.
+static void enqueue_candidate_space(hipStream_t stream, const LatticeView& lat, const PackedCorners& ct,
+                                    const int32_t* pairs, int64_t n_pairs, const int64_t* n_dev, const int64_t* grid_off,
+                                    int n_grids, const GridDesc* d_grids, double tet_mass, int precision, void* slab,
+                                    CandidateSpace& cs) {
     char* w = static_cast<char*>(slab);
     const int64_t nblk = (n_pairs + kScanBlock - 1) / kScanBlock;
     cs.pd = reinterpret_cast<PairData*>(w);
@@ -777,14 +831,26 @@ adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, c
     cs.n_pairs = n_pairs;
     cs.n_cand = 0;
     cs.n_tiles = 0;
-    if (n_pairs == 0) return ADECO_OK;
-    if (pair_grid) hipLaunchKernelGGL(k_pair_grid, dim3(unsigned(n_grids)), dim3(256), 0, stream, grid_off, pair_grid);
+    if (n_pairs == 0) return;
+    if (pair_grid)
+        hipLaunchKernelGGL(k_pair_grid, dim3(unsigned(n_grids)), dim3(256), 0, stream, grid_off, n_pairs, pair_grid);
     hipLaunchKernelGGL(k_pair_setup, dim3(unsigned((n_pairs + 255) / 256)), dim3(256), 0, stream, lat, ct, pairs, n_pairs,
-                       grid_off, n_grids, pair_grid, d_grids, tet_mass, precision == ADECO_PREC_FP64 ? kMaxSeg : 1, cs.cand, cs.pd);
-    hipLaunchKernelGGL(k_scan_local, dim3(unsigned(nblk)), dim3(256), 0, stream, cs.cand, n_pairs, cs.cand_off, cs.block_sum);
+                       n_dev, grid_off, n_grids, pair_grid, d_grids, tet_mass, precision == ADECO_PREC_FP64 ? kMaxSeg : 1,
+                       cs.cand, cs.pd);
+    hipLaunchKernelGGL(k_scan_local, dim3(unsigned(nblk)), dim3(256), 0, stream, cs.cand, n_pairs, n_dev, cs.cand_off,
+                       cs.block_sum);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, stream, cs.block_sum, nblk, cs.total);
     hipLaunchKernelGGL(k_scan_add, dim3(unsigned((n_pairs + 1 + 255) / 256)), dim3(256), 0, stream, cs.cand_off, n_pairs,
-                       cs.block_sum, cs.total);
+                       n_dev, cs.block_sum, cs.total);
+}
+
+adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, const PackedCorners& ct,
+                                   const int32_t* pairs, int64_t n_pairs, const int64_t* grid_off, int n_grids,
+                                   const GridDesc* d_grids, double tet_mass, int precision, void* slab,
+                                   CandidateSpace& cs) {
+    enqueue_candidate_space(stream, lat, ct, pairs, n_pairs, nullptr, grid_off, n_grids, d_grids, tet_mass, precision,
+                            slab, cs);
+    if (n_pairs == 0) return ADECO_OK;
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(&cs.n_cand, cs.total, 8, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -797,9 +863,23 @@ adeco_status build_tile_map(hipStream_t stream, CandidateSpace& cs, int32_t* til
     cs.tile_first = tiles;
     if (cs.n_pairs == 0) return ADECO_OK;
     hipLaunchKernelGGL(k_tile_map, dim3(unsigned((cs.n_pairs + 1 + 255) / 256)), dim3(256), 0, stream, cs.cand_off,
-                       cs.n_pairs, tiles, cs.n_tiles);
+                       cs.n_pairs, (const int64_t*)nullptr, (const int64_t*)nullptr, tiles, cs.n_tiles);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? ADECO_OK : hip_fail(e, "tile map");
+}
+
+hipError_t enqueue_candidate_space_dev(hipStream_t stream, const LatticeView& lat, const PackedCorners& ct,
+                                       const int32_t* pairs, int64_t pair_cap, const int64_t* n_dev,
+                                       const int64_t* grid_off, int n_grids, const GridDesc* d_grids, double tet_mass,
+                                       int precision, void* slab, int32_t* tiles, int64_t tile_cap, int64_t* sizes,
+                                       unsigned* overflow, CandidateSpace& cs) {
+    enqueue_candidate_space(stream, lat, ct, pairs, pair_cap, n_dev, grid_off, n_grids, d_grids, tet_mass, precision, slab,
+                            cs);
+    hipLaunchKernelGGL(k_space_sizes, dim3(1), dim3(1), 0, stream, cs.cand_off, n_dev, pair_cap, tile_cap, sizes, overflow);
+    cs.tile_first = tiles;
+    hipLaunchKernelGGL(k_tile_map, dim3(unsigned((pair_cap + 1 + 255) / 256)), dim3(256), 0, stream, cs.cand_off, pair_cap,
+                       n_dev, (const int64_t*)sizes, tiles, tile_cap);
+    return hipGetLastError();
 }
 
 hipError_t launch_deposit_tiles(hipStream_t stream, int precision, const TileSource& src, unsigned* queue_counter,

@@ -56,6 +56,7 @@ struct TileSource {
     const GridDesc* grids;
     double* grid_buf;
     double tet_mass;
+    const int64_t* dev_sizes = nullptr;  // when set, n_cand and n_tiles are dev_sizes[0], [1] (device)
 };
 
 // Device work arrays of one candidate space (allocated from one slab).
@@ -96,6 +97,22 @@ adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, c
                                    CandidateSpace& cs);
 // Writes the tile map into `tiles` (tile_map_bytes(cs.n_cand) bytes).
 adeco_status build_tile_map(hipStream_t stream, CandidateSpace& cs, int32_t* tiles);
+
+// The same without host reads, for graph capture: the pair count is *n_dev (at most pair_cap, the
+// slab's capacity: candidate_space_bytes(pair_cap)); pair setup, scan, sizes (sizes[0] = the
+// candidates, sizes[1] = the tiles) and the tile map (tile_cap + 1 entries) are enqueued.  A pair
+// count above pair_cap or a tile count above tile_cap sets *overflow and leaves no tiles.
+hipError_t enqueue_candidate_space_dev(hipStream_t stream, const LatticeView& lat, const PackedCorners& ct,
+                                       const int32_t* pairs, int64_t pair_cap, const int64_t* n_dev,
+                                       const int64_t* grid_off, int n_grids, const GridDesc* d_grids, double tet_mass,
+                                       int precision, void* slab, int32_t* tiles, int64_t tile_cap, int64_t* sizes,
+                                       unsigned* overflow, CandidateSpace& cs);
+
+// Exclusive scan of n uint32 counts into int64 offsets out[0..n] (out[n] = the total); block_sum
+// holds scan_block_words(n) words, total one.
+hipError_t launch_scan_u32(const unsigned* in, int64_t n, int64_t* out, int64_t* block_sum, int64_t* total,
+                           hipStream_t stream);
+size_t scan_block_words(int64_t n);
 
 // Persistent deposit over one candidate space (built for the same precision); counters[0] += exact
 // recomputes, counters[1] += partial candidates.  queue_counter must be zero.

@@ -12,6 +12,7 @@
 #include "deposit_tet.hpp"
 #include "kernels.hpp"
 #include "lattice_impl.hpp"
+#include "octree_impl.hpp"
 #include "runtime.hpp"
 
 namespace adeco {
@@ -215,6 +216,33 @@ adeco_status adeco_lattice_create(int device_id, const float* x, const float* y,
     }
     *out = L;
     return ADECO_OK;
+}
+
+adeco_status adeco_lattice_update(adeco_lattice L, const float* x, const float* y, const float* z) {
+    if (!L || !x || !y || !z) return fail(ADECO_EINVAL, "lattice_update(): null pointer argument");
+    const int64_t n = L->n_vertices;
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]))
+            return fail(ADECO_EINVAL, "lattice_update(): non-finite vertex coordinate");
+    DeviceState& ds = *L->ds;
+    std::lock_guard<std::mutex> lk(ds.mutex);
+    hipError_t e = hipSetDevice(L->device);
+    const size_t bytes = size_t(n) * sizeof(float);
+    if (e == hipSuccess) e = hipMemcpyAsync(L->d_x, x, bytes, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(L->d_y, y, bytes, hipMemcpyHostToDevice, ds.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(L->d_z, z, bytes, hipMemcpyHostToDevice, ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "lattice update");
+    if (!L->meta_box.empty()) {
+        // the meta-cell boxes that ROI selection culls with follow the new positions
+        const size_t M = L->meta_box.size() / 6;
+        DevBuf box, cells;
+        adeco_status st;
+        if ((st = box.reserve(M * 6 * sizeof(float))) || (st = cells.reserve(M * sizeof(unsigned)))) return st;
+        if ((st = octree_meta_bbox(L, L->meta_dims, box.as<float>(), cells.as<unsigned>()))) return st;
+        e = hipMemcpyAsync(L->meta_box.data(), box.ptr, M * 6 * sizeof(float), hipMemcpyDeviceToHost, ds.stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
+    return e == hipSuccess ? ADECO_OK : hip_fail(e, "lattice update");
 }
 
 adeco_status adeco_lattice_destroy(adeco_lattice L) {

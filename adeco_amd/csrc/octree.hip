@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/adeco.h"
+#include "batched_impl.hpp"
 #include "deposit_tet.hpp"
 #include "kernels.hpp"
 #include "lattice_impl.hpp"
@@ -124,16 +125,13 @@ __global__ void k_refine_level(const float* __restrict__ mbox, const unsigned* _
 // ---- per-tet binning into leaves (tree descent with a bounded explicit stack) ----
 constexpr int kMaxStack = 8 * kMaxOctreeDepth + 8;
 
-// (the tets of `ids[0..n_tets)`, the tets the leaf-map pass could not take)
 template <bool kFill>
-__global__ void __launch_bounds__(256) k_bin_tets(LatticeView lat, PackedCorners ct, int64_t n_tets,
-                                                  const int32_t* __restrict__ ids,
-                                                  const float* __restrict__ node_box, const int32_t* __restrict__ first_child,
-                                                  const int32_t* __restrict__ leaf_index, int* __restrict__ leaf_count,
-                                                  const int64_t* __restrict__ leaf_offset, int32_t* __restrict__ pairs) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n_tets) return;
-    const int64_t t = ids[i];
+__device__ __forceinline__ void bin_one_by_descent(const LatticeView& lat, const PackedCorners& ct, int64_t t,
+                                                   const float* __restrict__ node_box,
+                                                   const int32_t* __restrict__ first_child,
+                                                   const int32_t* __restrict__ leaf_index, int* __restrict__ leaf_count,
+                                                   const int64_t* __restrict__ leaf_offset, int32_t* __restrict__ pairs,
+                                                   int64_t pair_cap) {
     V3<double> v[4];
     lattice_tet_vertices(lat, ct, t, v);
     // the tet AABB in float is exact (vertices are float)
@@ -158,12 +156,27 @@ __global__ void __launch_bounds__(256) k_bin_tets(LatticeView lat, PackedCorners
         if (fc < 0) {
             const int leaf = leaf_index[nd];
             const int pos = atomicAdd(leaf_count + leaf, 1);
-            if (kFill) pairs[leaf_offset[leaf] + pos] = int32_t(t);
+            if (kFill && leaf_offset[leaf] + pos < pair_cap) pairs[leaf_offset[leaf] + pos] = int32_t(t);
             continue;
         }
         for (int c = 0; c < 8 && sp < kMaxStack; ++c)
             if (fstrict(tb, node_box + 6 * (fc + c))) stack[sp++] = fc + c;
     }
+}
+
+// (the tets of `ids[0..n)`, the tets the leaf-map pass could not take; n = n_tets, or *n_dev when
+// given, grid-stride; fill writes only pairs below pair_cap)
+template <bool kFill>
+__global__ void __launch_bounds__(256) k_bin_tets(LatticeView lat, PackedCorners ct, int64_t n_tets,
+                                                  const int* __restrict__ n_dev, const int32_t* __restrict__ ids,
+                                                  const float* __restrict__ node_box, const int32_t* __restrict__ first_child,
+                                                  const int32_t* __restrict__ leaf_index, int* __restrict__ leaf_count,
+                                                  const int64_t* __restrict__ leaf_offset, int32_t* __restrict__ pairs,
+                                                  int64_t pair_cap) {
+    const int64_t n = n_dev ? int64_t(*n_dev) : n_tets;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        bin_one_by_descent<kFill>(lat, ct, ids[i], node_box, first_child, leaf_index, leaf_count, leaf_offset, pairs,
+                                  pair_cap);
 }
 
 __global__ void k_iota(int32_t* __restrict__ out, int64_t n) {
@@ -182,13 +195,6 @@ __global__ void k_iota(int32_t* __restrict__ out, int64_t n) {
 // are aggregated per wave: one per distinct leaf among the wave's lanes.
 constexpr int kMapCells = 8;
 
-struct LeafMap {
-    const int32_t* map;       // F^3 leaf indices, i fastest
-    const float* leaf_box;    // per leaf (DFS order), widened as the node boxes
-    int F;
-    double ll[3];
-    double inv_h;
-};
 
 __device__ __forceinline__ unsigned wave_rank(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u));
@@ -202,8 +208,8 @@ template <bool kFill>
 __global__ void __launch_bounds__(256) k_bin_map(LatticeView lat, PackedCorners ct, int64_t n_tets, LeafMap lm,
                                                  const float* __restrict__ root_box, int* __restrict__ leaf_count,
                                                  const int64_t* __restrict__ leaf_offset, int32_t* __restrict__ pairs,
-                                                 int32_t* __restrict__ overflow, int* __restrict__ n_overflow,
-                                                 int32_t* __restrict__ tleaf) {
+                                                 int64_t pair_cap, int32_t* __restrict__ overflow,
+                                                 int* __restrict__ n_overflow, int32_t* __restrict__ tleaf) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     int leaf[kMapCells];
@@ -290,7 +296,10 @@ __global__ void __launch_bounds__(256) k_bin_map(LatticeView lat, PackedCorners 
             int base = 0;
             if (lane == src) base = atomicAdd(leaf_count + L, __popcll(m));
             base = __shfl(base, src, 64);
-            if (kFill && leaf[c] == L) pairs[leaf_offset[L] + base + int(wave_rank(m))] = int32_t(t);
+            if (kFill && leaf[c] == L) {
+                const int64_t at = leaf_offset[L] + base + int(wave_rank(m));
+                if (at < pair_cap) pairs[at] = int32_t(t);
+            }
             pending &= ~m;
         }
     }
@@ -351,7 +360,11 @@ adeco_status octree_refine_level(DeviceState& ds, const float* d_box, const unsi
     return e == hipSuccess ? ADECO_OK : hip_fail(e, "octree level refinement");
 }
 
-adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
+namespace {
+__global__ void k_set_int(int* __restrict__ p, int v) { *p = v; }
+}  // namespace
+
+adeco_status bin_plan(adeco_octree_s* T, int decomposition, BinPlan& p) {
     adeco_lattice_s* L = T->L;
     DeviceState& ds = *L->ds;
     const int64_t n_nodes = int64_t(T->nodes.size());
@@ -371,28 +384,26 @@ adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
         first[i] = T->nodes[i].first_child;
     }
     for (int64_t l = 0; l < n_leaves; ++l) leaf_index[T->leaves_dfs[l]] = int32_t(l);
+    const int64_t n_tets = 6 * int64_t(L->np[0] - 1) * (L->np[1] - 1) * (L->np[2] - 1);
     adeco_status st;
     if ((st = T->d_node_box.reserve(boxes.size() * 4)) || (st = T->d_first.reserve(n_nodes * 4)) ||
         (st = T->d_leaf_index.reserve(n_nodes * 4)) || (st = T->d_leaf_count.reserve(n_leaves * 4 + 4)) ||
-        (st = T->d_leaf_offset.reserve((n_leaves + 1) * 8)))
+        (st = T->d_leaf_offset.reserve((n_leaves + 1) * 8)) || (st = T->d_overflow.reserve(size_t(n_tets) * 4 + 8)))
         return st;
     hipError_t e = hipMemcpyAsync(T->d_node_box.ptr, boxes.data(), boxes.size() * 4, hipMemcpyHostToDevice, ds.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(T->d_first.ptr, first.data(), n_nodes * 4, hipMemcpyHostToDevice, ds.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(T->d_leaf_index.ptr, leaf_index.data(), n_nodes * 4, hipMemcpyHostToDevice, ds.stream);
-    if (e == hipSuccess) e = hipMemsetAsync(T->d_leaf_count.ptr, 0, n_leaves * 4, ds.stream);
     if (e != hipSuccess) return hip_fail(e, "binning setup");
-    const int64_t n_tets = 6 * int64_t(L->np[0] - 1) * (L->np[1] - 1) * (L->np[2] - 1);
-    const LatticeView lv{L->d_x, L->d_y, L->d_z, {L->np[0], L->np[1], L->np[2]}};
-    const PackedCorners ct = pack_corners(corner_table(decomposition));
-    const unsigned nb = unsigned((n_tets + 255) / 256);
     // the leaf map on the finest leaf level, when that level is at most 7 (F <= 128)
     int lmax = 0;
     for (int64_t l = 0; l < n_leaves; ++l) lmax = std::max(lmax, T->nodes[T->leaves_dfs[l]].level);
-    const bool use_map = lmax <= 7;
-    LeafMap lm{};
-    if (use_map) {
+    p = BinPlan();
+    p.use_map = lmax <= 7;
+    std::vector<int32_t> map;
+    std::vector<float> leaf_box;
+    if (p.use_map) {
         const int F = 1 << lmax;
-        std::vector<int32_t> map(size_t(F) * F * F, -1);
+        map.assign(size_t(F) * F * F, -1);
         for (int64_t l = 0; l < n_leaves; ++l) {
             const ONode& n = T->nodes[T->leaves_dfs[l]];
             const int w = 1 << (lmax - n.level);
@@ -402,57 +413,82 @@ adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
         }
         for (int32_t m : map)
             if (m < 0) return fail(ADECO_EINTERNAL, "binning: the leaves do not tile the root");
-        std::vector<float> leaf_box(6 * size_t(n_leaves));
+        leaf_box.resize(6 * size_t(n_leaves));
         for (int64_t l = 0; l < n_leaves; ++l)
             std::memcpy(&leaf_box[6 * l], &boxes[6 * size_t(T->leaves_dfs[l])], 6 * sizeof(float));
-        if ((st = T->d_leaf_map.reserve(map.size() * 4 + leaf_box.size() * 4)) ||
-            (st = T->d_overflow.reserve(size_t(n_tets) * 4 + 8)))
+        if ((st = T->d_leaf_map.reserve(map.size() * 4 + leaf_box.size() * 4)) || (st = T->d_tleaf.reserve(size_t(n_tets) * 4)))
             return st;
         e = hipMemcpyAsync(T->d_leaf_map.ptr, map.data(), map.size() * 4, hipMemcpyHostToDevice, ds.stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(T->d_leaf_map.as<char>() + map.size() * 4, leaf_box.data(), leaf_box.size() * 4,
                                hipMemcpyHostToDevice, ds.stream);
-        if (e == hipSuccess) e = hipMemsetAsync(T->d_overflow.ptr, 0, 8, ds.stream);
         if (e != hipSuccess) return hip_fail(e, "leaf map upload");
-        lm.map = T->d_leaf_map.as<int32_t>();
-        lm.leaf_box = reinterpret_cast<const float*>(T->d_leaf_map.as<char>() + map.size() * 4);
-        lm.F = F;
+        p.lm.map = T->d_leaf_map.as<int32_t>();
+        p.lm.leaf_box = reinterpret_cast<const float*>(T->d_leaf_map.as<char>() + map.size() * 4);
+        p.lm.F = F;
         const double h = double(T->nodes[0].ext) / double(F);
-        for (int a = 0; a < 3; ++a) lm.ll[a] = double(T->nodes[0].ll[a]);
-        lm.inv_h = 1.0 / h;
+        for (int a = 0; a < 3; ++a) p.lm.ll[a] = double(T->nodes[0].ll[a]);
+        p.lm.inv_h = 1.0 / h;
+        p.tleaf = T->d_tleaf.as<int32_t>();
     }
-    if (use_map && (st = T->d_tleaf.reserve(size_t(n_tets) * 4))) return st;
-    int32_t* d_tleaf = use_map ? T->d_tleaf.as<int32_t>() : nullptr;
-    int* d_nover = use_map ? T->d_overflow.as<int>() : nullptr;
-    int32_t* d_over = use_map ? reinterpret_cast<int32_t*>(T->d_overflow.as<char>() + 8) : nullptr;
-    int n_over = 0;
-    if (use_map) {
-        hipLaunchKernelGGL(k_bin_map<false>, dim3(nb), dim3(256), 0, ds.stream, lv, ct, n_tets, lm, T->d_node_box.as<float>(),
-                           T->d_leaf_count.as<int>(), (const int64_t*)nullptr, (int32_t*)nullptr, d_over, d_nover, d_tleaf);
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(&n_over, d_nover, 4, hipMemcpyDeviceToHost, ds.stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
-        if (e != hipSuccess) return hip_fail(e, "binning count pass");
+    // the host vectors above are the sources of the copies
+    e = hipStreamSynchronize(ds.stream);
+    if (e != hipSuccess) return hip_fail(e, "binning setup");
+    p.lv = LatticeView{L->d_x, L->d_y, L->d_z, {L->np[0], L->np[1], L->np[2]}};
+    p.ct = pack_corners(corner_table(decomposition));
+    p.n_tets = n_tets;
+    p.n_leaves = int(n_leaves);
+    p.node_box = T->d_node_box.as<float>();
+    p.first = T->d_first.as<int32_t>();
+    p.leaf_index = T->d_leaf_index.as<int32_t>();
+    p.leaf_count = T->d_leaf_count.as<int>();
+    p.n_over = T->d_overflow.as<int>();
+    p.over = reinterpret_cast<int32_t*>(T->d_overflow.as<char>() + 8);
+    return ADECO_OK;
+}
+
+// grid of the descent passes: grid-stride over a count that only the device knows
+static unsigned descent_blocks(int64_t n_tets) { return unsigned(std::min<int64_t>((n_tets + 255) / 256, 8192)); }
+
+hipError_t bin_count_async(const BinPlan& p, hipStream_t stream) {
+    const unsigned nb = unsigned((p.n_tets + 255) / 256);
+    hipError_t e = launch_zero_u32(reinterpret_cast<unsigned*>(p.leaf_count), p.n_leaves, stream);
+    if (e != hipSuccess) return e;
+    if (p.use_map) {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, stream, p.n_over, 0);
+        hipLaunchKernelGGL(k_bin_map<false>, dim3(nb), dim3(256), 0, stream, p.lv, p.ct, p.n_tets, p.lm, p.node_box,
+                           p.leaf_count, (const int64_t*)nullptr, (int32_t*)nullptr, int64_t(0), p.over, p.n_over, p.tleaf);
     } else {
-        if ((st = T->d_overflow.reserve(size_t(n_tets) * 4 + 8))) return st;
-        d_over = reinterpret_cast<int32_t*>(T->d_overflow.as<char>() + 8);
-        hipLaunchKernelGGL(k_iota, dim3(nb), dim3(256), 0, ds.stream, d_over, n_tets);
-        n_over = int(n_tets);
+        hipLaunchKernelGGL(k_iota, dim3(nb), dim3(256), 0, stream, p.over, p.n_tets);
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, stream, p.n_over, int(p.n_tets));
     }
-    auto descent = [&](bool fill) {
-        if (n_over == 0) return;
-        const unsigned ob = unsigned((int64_t(n_over) + 255) / 256);
-        if (fill)
-            hipLaunchKernelGGL(k_bin_tets<true>, dim3(ob), dim3(256), 0, ds.stream, lv, ct, int64_t(n_over), (const int32_t*)d_over,
-                               T->d_node_box.as<float>(), T->d_first.as<int32_t>(), T->d_leaf_index.as<int32_t>(),
-                               T->d_leaf_count.as<int>(), T->d_leaf_offset.as<int64_t>(), T->d_pairs.as<int32_t>());
-        else
-            hipLaunchKernelGGL(k_bin_tets<false>, dim3(ob), dim3(256), 0, ds.stream, lv, ct, int64_t(n_over), (const int32_t*)d_over,
-                               T->d_node_box.as<float>(), T->d_first.as<int32_t>(), T->d_leaf_index.as<int32_t>(),
-                               T->d_leaf_count.as<int>(), (const int64_t*)nullptr, (int32_t*)nullptr);
-    };
-    descent(false);
-    e = hipGetLastError();
+    hipLaunchKernelGGL(k_bin_tets<false>, dim3(descent_blocks(p.n_tets)), dim3(256), 0, stream, p.lv, p.ct, p.n_tets,
+                       (const int*)p.n_over, (const int32_t*)p.over, p.node_box, p.first, p.leaf_index, p.leaf_count,
+                       (const int64_t*)nullptr, (int32_t*)nullptr, int64_t(0));
+    return hipGetLastError();
+}
+
+hipError_t bin_fill_async(const BinPlan& p, const int64_t* d_leaf_offset, int32_t* d_pairs, int64_t pair_cap,
+                          hipStream_t stream) {
+    const unsigned nb = unsigned((p.n_tets + 255) / 256);
+    hipError_t e = launch_zero_u32(reinterpret_cast<unsigned*>(p.leaf_count), p.n_leaves, stream);
+    if (e != hipSuccess) return e;
+    if (p.use_map)
+        hipLaunchKernelGGL(k_bin_map<true>, dim3(nb), dim3(256), 0, stream, p.lv, p.ct, p.n_tets, p.lm, p.node_box,
+                           p.leaf_count, d_leaf_offset, d_pairs, pair_cap, (int32_t*)nullptr, (int*)nullptr, p.tleaf);
+    hipLaunchKernelGGL(k_bin_tets<true>, dim3(descent_blocks(p.n_tets)), dim3(256), 0, stream, p.lv, p.ct, p.n_tets,
+                       (const int*)p.n_over, (const int32_t*)p.over, p.node_box, p.first, p.leaf_index, p.leaf_count,
+                       d_leaf_offset, d_pairs, pair_cap);
+    return hipGetLastError();
+}
+
+adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
+    DeviceState& ds = *T->L->ds;
+    BinPlan p;
+    adeco_status st = bin_plan(T, decomposition, p);
+    if (st) return st;
+    const int64_t n_leaves = p.n_leaves;
+    hipError_t e = bin_count_async(p, ds.stream);
     std::vector<int> cnt(n_leaves);
     if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), T->d_leaf_count.ptr, n_leaves * 4, hipMemcpyDeviceToHost, ds.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
@@ -462,14 +498,7 @@ adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition) {
     const int64_t total = T->leaf_tet_offset[n_leaves];
     if ((st = T->d_pairs.reserve(size_t(total) * 4 + 4))) return st;
     e = hipMemcpyAsync(T->d_leaf_offset.ptr, T->leaf_tet_offset.data(), (n_leaves + 1) * 8, hipMemcpyHostToDevice, ds.stream);
-    if (e == hipSuccess) e = hipMemsetAsync(T->d_leaf_count.ptr, 0, n_leaves * 4, ds.stream);
-    if (e != hipSuccess) return hip_fail(e, "binning offsets");
-    if (use_map)
-        hipLaunchKernelGGL(k_bin_map<true>, dim3(nb), dim3(256), 0, ds.stream, lv, ct, n_tets, lm, T->d_node_box.as<float>(),
-                           T->d_leaf_count.as<int>(), T->d_leaf_offset.as<int64_t>(), T->d_pairs.as<int32_t>(),
-                           (int32_t*)nullptr, (int*)nullptr, d_tleaf);
-    descent(true);
-    e = hipGetLastError();
+    if (e == hipSuccess) e = bin_fill_async(p, T->d_leaf_offset.as<int64_t>(), T->d_pairs.as<int32_t>(), total, ds.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ds.stream);
     if (e != hipSuccess) return hip_fail(e, "binning fill pass");
     T->binned_decomposition = decomposition;

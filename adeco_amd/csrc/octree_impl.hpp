@@ -97,4 +97,39 @@ adeco_status octree_refine_level(DeviceState& ds, const float* d_box, const unsi
                                  std::vector<unsigned long long>& child_count);
 adeco_status octree_bin_tets(adeco_octree_s* T, int decomposition);
 
+// The leaf of every cell of the finest leaf level, for the per-tet binning.
+struct LeafMap {
+    const int32_t* map;       // F^3 leaf indices, i fastest
+    const float* leaf_box;    // per leaf (DFS order), widened as the node boxes
+    int F;
+    double ll[3];
+    double inv_h;
+};
+
+// Everything the binning passes need, resolved once per octree and decomposition (node boxes,
+// leaf map and work buffers uploaded by bin_plan); the passes then run with every size on the
+// device, so they can be captured in a graph and replayed on moved lattice positions.
+struct BinPlan {
+    LatticeView lv{};
+    PackedCorners ct{};
+    int64_t n_tets = 0;
+    int n_leaves = 0;
+    bool use_map = false;
+    LeafMap lm{};
+    const float* node_box = nullptr;
+    const int32_t* first = nullptr;
+    const int32_t* leaf_index = nullptr;
+    int* leaf_count = nullptr;  // n_leaves counters
+    int32_t* over = nullptr;    // tets left to the descent, and their count
+    int* n_over = nullptr;
+    int32_t* tleaf = nullptr;   // per tet: its single leaf, or a recompute mark
+};
+adeco_status bin_plan(adeco_octree_s* T, int decomposition, BinPlan& p);
+// Count pass: leaf_count[l] = the tets whose AABB strictly overlaps leaf l.
+hipError_t bin_count_async(const BinPlan& p, hipStream_t stream);
+// Fill pass: the tets of leaf l at d_pairs[d_leaf_offset[l] ...]; entries at or beyond pair_cap
+// are not written.
+hipError_t bin_fill_async(const BinPlan& p, const int64_t* d_leaf_offset, int32_t* d_pairs, int64_t pair_cap,
+                          hipStream_t stream);
+
 }  // namespace adeco

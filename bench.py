@@ -59,10 +59,11 @@ TET_MASS = 1.0 / 6.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (the roof of an FP64 VALU-bound kernel)
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of k_deposit_waves<FP64, false> on the 256^3 infall workload at 16^3-cell
-# leaves: rocprofv3 FETCH_SIZE + WRITE_SIZE (KiB per dispatch, each scaled by what it reads for a
-# 1 GiB copy), profiles/deposit_waves_c3_pmc.md
-TRAFFIC_BYTES = {"c3": None, "c2": None}
+# HBM bytes per launch of k_deposit_waves<FP64, false> at 16^3-cell leaves: rocprofv3 FETCH_SIZE
+# and WRITE_SIZE (KiB per dispatch) scaled by what each reads for a 1 GiB copy (2.0 and 1.0),
+# profiles/deposit_waves_c3_pmc.md (256^3 infall workload) and profiles/deposit_waves_c2_pmc.md
+# (128^3)
+TRAFFIC_BYTES = {"c3": 14281590.94 * 2 * 1024 + 18129937.53 * 1024, "c2": None}
 
 
 def log(*a):
@@ -743,11 +744,21 @@ def headline(args):
             g.launch()
             gms = [g.launch() for _ in range(max(1, args.steps))]
             gerr = max_rel_err(g.leaf_grids(), grids64, 1e-8 * TET_MASS)
-        del grids64
         extra["graph_replay_ms"] = float(np.mean(gms))
         extra["graph_replay_tets_per_s"] = lat.num_tets / (float(np.mean(gms)) * 1e-3)
         extra["graph_vs_eager_max_rel_err"] = gerr
         log(f"[bench] graph replay: {np.mean(gms):.2f} ms ({extra['graph_replay_tets_per_s']:.3e} tets/s), err {gerr:.1e}")
+        # the whole step (binning, pair setup, scans, tile map, deposit) as one graph with device-side
+        # sizes, replayed: the eager step's work without its host round trips
+        with A.DepositGraph(tree, DIMS, decomp, TET_MASS, A.PREC_FP64, step=True) as g:
+            g.launch()
+            sms = [g.launch() for _ in range(max(1, args.steps))]
+            serr = max_rel_err(g.leaf_grids(), grids64, 1e-8 * TET_MASS)
+        extra["step_graph_ms"] = float(np.mean(sms))
+        extra["step_graph_tets_per_s"] = lat.num_tets / (float(np.mean(sms)) * 1e-3)
+        extra["step_graph_vs_eager_max_rel_err"] = serr
+        log(f"[bench] step graph replay: {np.mean(sms):.2f} ms ({extra['step_graph_tets_per_s']:.3e} tets/s), err {serr:.1e}")
+        del grids64
         # the whole one-rank workflow per step: octree refinement + binning + deposit + leaf grids to host
         wf = []
         for _ in range(3):

@@ -125,6 +125,10 @@ typedef struct adeco_lattice_s* adeco_lattice;
 adeco_status adeco_lattice_create(int device_id, const float* x, const float* y, const float* z,
                                   int npx, int npy, int npz, adeco_lattice* out);
 adeco_status adeco_lattice_destroy(adeco_lattice lat);
+/* Overwrites the resident vertex positions with a new state of the same lattice (the next snapshot
+ * of a simulation); octrees, graphs and servers on the lattice see the new positions from their
+ * next deposit.  Not concurrently with other calls on the lattice. */
+adeco_status adeco_lattice_update(adeco_lattice lat, const float* x, const float* y, const float* z);
 /* Float min/max over all vertices (AdPointDataReaders.h:90-151), computed on the device. */
 adeco_status adeco_lattice_bbox(adeco_lattice lat, float* bbox6);
 
@@ -237,6 +241,15 @@ adeco_status adeco_autotune_batched(adeco_octree tree, int dims, int decompositi
 typedef struct adeco_deposit_graph_s* adeco_deposit_graph;
 adeco_status adeco_deposit_graph_create(adeco_octree tree, int dims, int decomposition, double tet_mass, int precision,
                                         adeco_deposit_graph* out);
+/* hipGraph of the whole leaf-deposit step (no reference counterpart): per-leaf binning of the
+ * lattice's current positions, pair setup, candidate scan, tile map, grid clear and the persistent
+ * deposit, with every size kept on the device, so replays follow later positions of the lattice
+ * (adeco_lattice_update) without a new capture; the leaves stay the captured octree's.  Buffers are
+ * sized for slack (1..16) times the capture's pairs and tiles; a replay that needs more deposits
+ * nothing and returns ADECO_EOVERFLOW (create the graph again).  Launched, read and destroyed like
+ * the deposit graph. */
+adeco_status adeco_deposit_graph_create_step(adeco_octree tree, int dims, int decomposition, double tet_mass,
+                                             int precision, double slack, adeco_deposit_graph* out);
 adeco_status adeco_deposit_graph_launch(adeco_deposit_graph graph);
 adeco_status adeco_deposit_graph_grids(adeco_deposit_graph graph, int64_t first_leaf, int64_t n_leaves, double* out);
 adeco_status adeco_deposit_graph_destroy(adeco_deposit_graph graph);

@@ -84,11 +84,14 @@ def test_batched_is_repeatable_and_dims_change():
     assert abs(a.sum() - b.sum()) <= 1e-9 * a.sum()
 
 
-def test_graph_replays_match_eager():
+def test_graph_replays_match_oracle():
     """The captured graph (grid clear, work-queue reset, persistent launch) replayed several times,
-    with eager deposits of other sizes in between, gives the eager batched result each time."""
+    with eager deposits of other sizes in between: every replay matches the oracle per leaf to 1e-6
+    (floor 1e-8 of a tet mass) and the eager batched result to 1e-12."""
     lat = make_lattice(20, 2.5, seed=12)
-    with A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 1200, 4) as tree:
+    with O.cell_scaling(O.EXACT), A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 1200, 4) as tree:
+        tree.bin(A.DECOMP_KUHN)
+        want = oracle_leaves(lat, tree, A.DECOMP_KUHN, 12, range(tree.num_leaves))
         for prec in (A.PREC_FP64, A.PREC_MIXED):
             tree.deposit(12, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=A.MODE_BATCHED, precision=prec)
             eager = tree.leaf_grids()
@@ -97,9 +100,69 @@ def test_graph_replays_match_eager():
                     print(f"[graph] precision={prec} replay {rep}", flush=True)
                     g.launch()
                     got = g.leaf_grids()
+                    assert O.max_rel_err(got, want, tet_mass=TET_MASS) <= REL
                     assert O.max_rel_err(got, eager, tet_mass=TET_MASS) <= 1e-12
                     assert A.Octree.last_timings()["partials"] > 0
                     tree.deposit(8 + 2 * rep, decomposition=A.DECOMP_KUHN, tet_mass=TET_MASS, mode=A.MODE_BATCHED)
+
+
+def _moved(lat, scale=1.0, shift=0.0, noise=0.0, seed=7):
+    """The lattice's vertices scaled about their mean, shifted and jittered (a later snapshot)."""
+    from adeco_amd.lattice import Lattice
+    rng = np.random.default_rng(seed)
+    out = []
+    for v in (lat.x, lat.y, lat.z):
+        c = float(v.mean())
+        out.append(((v - c) * scale + c + shift + noise * rng.standard_normal(v.size)).astype(np.float32))
+    return Lattice(n=lat.n, box=lat.box, x=out[0], y=out[1], z=out[2])
+
+
+def _oracle_all_leaves(lat, boxes, dims):
+    """The oracle's deposit of every lattice tet onto every leaf grid (tets outside a leaf add
+    nothing, so no binning is assumed)."""
+    return np.array([O.deposit_lattice(lat, b, dims, decomp=A.DECOMP_KUHN, tet_mass=TET_MASS, precision=3, nthreads=8)
+                     for b in boxes])
+
+
+def test_step_graph_follows_the_lattice():
+    """The whole step as one graph (binning of the current positions, pair setup, scans, tile map,
+    deposit, all sizes on the device), created on one lattice state and replayed on others after
+    ResidentLattice.update: every replay matches the oracle's deposit of that state per leaf to 1e-6
+    (floor 1e-8 of a tet mass).  The states: the capture's, shifted by 0.3 Mpc/h, jittered by 1 Mpc/h
+    (3 % more (leaf, tet) pairs and 9 % more tiles than the capture, within the graph's 1.25x), and
+    the capture's again (equal to the first replay to 1e-12).  A state jittered by 2 Mpc/h needs 12 %
+    more pairs and 30 % more tiles: a graph with no slack reports ADECO_EOVERFLOW on it and deposits
+    nothing, and a later replay on the capture's state works."""
+    lat = make_lattice(20, 2.5, seed=12)
+    dims = 8
+    states = [("capture", lat), ("shifted", _moved(lat, shift=0.3)), ("jittered", _moved(lat, noise=1.0)),
+              ("capture again", lat)]
+    with O.cell_scaling(O.EXACT), A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, 1200, 4) as tree:
+        boxes = tree.leaf_boxes_f64(exact=True)
+        with A.DepositGraph(tree, dims, A.DECOMP_KUHN, TET_MASS, A.PREC_FP64, step=True, slack=1.25) as g:
+            first = None
+            for name, st in states:
+                print(f"[step graph] {name}", flush=True)
+                rl.update(st.x, st.y, st.z)
+                g.launch()
+                got = g.leaf_grids()
+                want = _oracle_all_leaves(st, boxes, dims)
+                assert O.max_rel_err(got, want, tet_mass=TET_MASS) <= REL, name
+                if first is None:
+                    first = got
+                elif name == "capture again":
+                    assert O.max_rel_err(got, first, tet_mass=TET_MASS) <= 1e-12
+        rl.update(lat.x, lat.y, lat.z)
+        with A.DepositGraph(tree, dims, A.DECOMP_KUHN, TET_MASS, A.PREC_FP64, step=True, slack=1.0) as g:
+            jit = _moved(lat, noise=2.0)
+            rl.update(jit.x, jit.y, jit.z)
+            with pytest.raises(A.AdRuntimeException) as ei:
+                g.launch()
+            assert ei.value.status == A.ADECO_EOVERFLOW
+            assert not g.leaf_grids().any()
+            rl.update(lat.x, lat.y, lat.z)
+            g.launch()
+            assert O.max_rel_err(g.leaf_grids(), first, tet_mass=TET_MASS) <= 1e-12
 
 
 def test_autotune_keeps_results():
