@@ -321,6 +321,30 @@ ADECO_HD R take(const Stage<R, 6>& st, int k, V3<R> out[4]) {
     return R(1);
 }
 
+// Moves the next piece of a stage into slots 0..3: pieces of a sequence are consecutive windows,
+// so the next one is the sequence shifted by one, except after the first piece of a corner stage,
+// whose second piece is window 3.  The loops over a stage's pieces read slots 0..3 only.
+template <class R>
+ADECO_HD void advance(Stage<R, 7>& st) {
+    const bool c = st.corner;
+    st.s[0] = sel(c, st.s[3], st.s[1]);
+    st.s[1] = sel(c, st.s[4], st.s[2]);
+    st.s[2] = sel(c, st.s[5], st.s[3]);
+    st.s[3] = sel(c, st.s[6], st.s[4]);
+    st.s[4] = st.s[5];
+}
+
+template <class R>
+ADECO_HD void advance(Stage<R, 6>& st) {
+    st.s[0] = st.s[1]; st.s[1] = st.s[2]; st.s[2] = st.s[3]; st.s[3] = st.s[4]; st.s[4] = st.s[5];
+}
+
+// the sign of piece k of a stage whose pieces are read from slots 0..3 (advance)
+template <class R>
+ADECO_HD R piece_sign(const Stage<R, 7>& st, int k) { return (k == 1 && st.corner) ? R(-1) : R(1); }
+template <class R>
+ADECO_HD R piece_sign(const Stage<R, 6>&, int) { return R(1); }
+
 // Which of the faces x>=0, x<=1, y>=0, y<=1 cut the tet (bit f set), or -1 if the tet lies
 // entirely outside one of them.
 template <class R>
@@ -380,7 +404,9 @@ ADECO_HD void clip_tree(const V3<R> t[4], int nact, bool xy, bool use_signed, Le
     const int ax1 = xy ? 1 : 0;
     const R c1 = xy ? R(0) : R(1);
     for (int k1 = 0; k1 < s1.count; ++k1) {
-        const R g1 = take(s1, k1, q1);
+        const R g1 = piece_sign(s1, k1);
+        for (int q = 0; q < 4; ++q) q1[q] = s1.s[q];
+        advance(s1);
         clip_stage(q1, ax1, c1, xy, s2, use_signed);
         if (cnt) cnt->stages += 1;
         if (nact == 2) {
@@ -388,7 +414,9 @@ ADECO_HD void clip_tree(const V3<R> t[4], int nact, bool xy, bool use_signed, Le
             continue;
         }
         for (int k2 = 0; k2 < s2.count; ++k2) {
-            const R g2 = g1 * take(s2, k2, q2);
+            const R g2 = g1 * piece_sign(s2, k2);
+            for (int q = 0; q < 4; ++q) q2[q] = s2.s[q];
+            advance(s2);
             clip_stage(q2, 1, R(0), true, s3, use_signed);
             if (cnt) cnt->stages += 1;
             if (nact == 3) {
@@ -396,7 +424,9 @@ ADECO_HD void clip_tree(const V3<R> t[4], int nact, bool xy, bool use_signed, Le
                 continue;
             }
             for (int k3 = 0; k3 < s3.count; ++k3) {
-                const R g3 = g2 * take(s3, k3, q3);
+                const R g3 = g2 * piece_sign(s3, k3);
+                for (int q = 0; q < 4; ++q) q3[q] = s3.s[q];
+                advance(s3);
                 clip_stage(q3, 1, R(1), false, s4, use_signed);
                 if (cnt) cnt->stages += 1;
                 stage_leaves(s4, g3, leaf, cnt);
