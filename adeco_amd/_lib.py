@@ -154,7 +154,7 @@ class ServeStats(ctypes.Structure):
     """adeco_serve_stats"""
     _fields_ = [("requests_done", ctypes.c_uint64), ("steps", ctypes.c_uint64), ("tiles", ctypes.c_int64),
                 ("tets", ctypes.c_int64), ("busy_s", ctypes.c_double), ("device_bytes", ctypes.c_uint64),
-                ("pending", ctypes.c_int64), ("active", ctypes.c_int64)]
+                ("pending", ctypes.c_int64), ("active", ctypes.c_int64), ("step_kernel_s", ctypes.c_double)]
 
 
 _lib = None

@@ -48,7 +48,7 @@ namespace adeco {
 
 namespace {
 
-static_assert(sizeof(adeco_serve_config) == 40 && sizeof(adeco_request_info) == 48 && sizeof(adeco_serve_stats) == 64,
+static_assert(sizeof(adeco_serve_config) == 40 && sizeof(adeco_request_info) == 48 && sizeof(adeco_serve_stats) == 72,
               "C-ABI struct layouts are fixed");
 
 constexpr int kMeta = 7;  // lattice cubes per meta-cell edge (AdTypeDefinitions.h:732-734)
@@ -328,6 +328,23 @@ struct adeco_server_s {
     uint64_t n_steps = 0, n_done = 0;
     int64_t tiles_done = 0, tets_done = 0;
     double busy_s = 0;
+    // device time of the step launches: an event pair recorded around each one, added up once the
+    // pair has run (under mu); spent pairs are reused
+    std::deque<std::pair<hipEvent_t, hipEvent_t>> step_events;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spare_events;
+    double step_kernel_s = 0;
+
+    void collect_step_times(bool wait) {
+        while (!step_events.empty()) {
+            auto ev = step_events.front();
+            if (wait) (void)hipEventSynchronize(ev.second);
+            else if (hipEventQuery(ev.second) != hipSuccess) break;
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) step_kernel_s += 1e-3 * double(ms);
+            step_events.pop_front();
+            spare_events.push_back(ev);
+        }
+    }
 
     size_t capacity_in_use() const {
         size_t b = 0;
@@ -358,6 +375,13 @@ struct adeco_server_s {
                 (void)hipEventDestroy(slot_done[i]);
                 slot_done[i] = nullptr;
             }
+        for (auto& ev : step_events) spare_events.push_back(ev);
+        step_events.clear();
+        for (auto& ev : spare_events) {
+            (void)hipEventDestroy(ev.first);
+            (void)hipEventDestroy(ev.second);
+        }
+        spare_events.clear();
         d_sources.free_all();
         d_ctr.free_all();
         if (stream) (void)hipStreamDestroy(stream);
@@ -529,9 +553,33 @@ void adeco_server_s::run_worker() {
             if (e == hipSuccess) e = hipEventRecord(entries_uploaded[buf], stream);
             if (e == hipSuccess) entries_in_flight[buf] = true;
             if (e == hipSuccess) e = launch_zero_u32(ctr, 16, stream);
+            std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+            if (e == hipSuccess) {
+                lk.lock();
+                if (!spare_events.empty()) {
+                    ev = spare_events.back();
+                    spare_events.pop_back();
+                }
+                lk.unlock();
+                if (!ev.first) {
+                    e = hipEventCreate(&ev.first);
+                    if (e == hipSuccess) e = hipEventCreate(&ev.second);
+                }
+            }
+            if (e == hipSuccess) e = hipEventRecord(ev.first, stream);
             if (e == hipSuccess)
                 e = launch_deposit_step(stream, cfg.precision, d_sources.as<TileSource>(), d_entries[buf].as<int2>(), n, ctr,
                                         counters, L->device);
+            if (e == hipSuccess) e = hipEventRecord(ev.second, stream);
+            lk.lock();
+            if (e == hipSuccess) step_events.push_back(ev);
+            else if (ev.first && ev.second) spare_events.push_back(ev);
+            else {
+                if (ev.first) (void)hipEventDestroy(ev.first);
+                if (ev.second) (void)hipEventDestroy(ev.second);
+            }
+            collect_step_times(false);
+            lk.unlock();
         }
         buf ^= 1;
         // requests whose last tile is in this step: density in FP32, then an event for the copier
@@ -718,6 +766,8 @@ adeco_status adeco_serve_stats_get(adeco_server S, adeco_serve_stats* out) {
     out->device_bytes = S->capacity_in_use();
     out->pending = int64_t(S->pending.size());
     out->active = int64_t(S->active.size());
+    S->collect_step_times(S->pending.empty() && S->active.empty() && S->to_copy.empty());
+    out->step_kernel_s = S->step_kernel_s;
     return ADECO_OK;
 }
 

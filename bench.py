@@ -290,6 +290,12 @@ def serve_study(rates, count, seed=5, max_inflight=16):
     log(f"[serve] lattice {SERVE_N}^3 generated in {time.perf_counter() - t0:.1f}s")
     unit_arr, rois, dims = serve_requests(count, seed)
     out = {"lattice": SERVE_N, "tets": lat.num_tets, "requests": count, "max_inflight": max_inflight, "rates": []}
+    # the reference algorithm's FLOPs of this request set (tests/golden/c5_flops.json: 60 requests, seed 5)
+    serve_flops = None
+    fx = os.path.join(ROOT, "tests", "golden", "c5_flops.json")
+    if count == 60 and seed == 5 and os.path.isfile(fx):
+        with open(fx) as f:
+            serve_flops = float(json.load(f)["flops"])
     with A.ResidentLattice.from_lattice(lat) as rl:
         # serial plain GPU service times (independent of the arrival rate)
         service, plain_keep, n_tets = [], {}, []
@@ -348,8 +354,14 @@ def serve_study(rates, count, seed=5, max_inflight=16):
             rec = {"rate_req_s": rate, "batched_p50_s": p50, "batched_p99_s": p99,
                    "batched_tets_per_s": tets / makespan, "batched_makespan_s": makespan, "steps": int(st["steps"]),
                    "step_tiles": int(st["tiles"]), "worker_busy_s": float(st["busy_s"]),
+                   "step_kernel_s": float(st["step_kernel_s"]),
                    "plain_serial_p50_s": q50, "plain_serial_p99_s": q99, "plain_serial_tets_per_s": tets / finish,
                    "plain_serial_makespan_s": finish, "batched_vs_plain_max_rel_err": err, "by_resolution": by_res}
+            if serve_flops is not None:
+                ach = serve_flops / float(st["step_kernel_s"]) / 1e12
+                rec["roofline"] = {"bound": "valu", "kernel": "k_deposit_step<FP64>", "achieved": ach,
+                                   "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                                   "flops": serve_flops, "kernel_s": float(st["step_kernel_s"])}
             log(f"[serve] {rate:g} req/s: batched p50 {p50 * 1e3:.1f} ms p99 {p99 * 1e3:.1f} ms "
                 f"({tets / makespan:.3e} tets/s); serial plain p50 {q50 * 1e3:.1f} ms p99 {q99 * 1e3:.1f} ms "
                 f"({tets / finish:.3e} tets/s); max rel diff {err:.2e}")

@@ -363,6 +363,8 @@ typedef struct {
     double busy_s;        /* time the worker spent in steps */
     uint64_t device_bytes; /* request buffers held */
     int64_t pending, active;
+    double step_kernel_s; /* device time of the finished step launches (k_deposit_step), from HIP
+                           * events recorded around each launch on the step stream */
 } adeco_serve_stats;
 /* Starts a server (a worker and a copier thread) on the lattice's device.  Requests are admitted
  * in arrival order while a slot is free and the memory budget allows; each step launch deposits an

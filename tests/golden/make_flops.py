@@ -15,8 +15,12 @@ contribution).
                  sample of leaves stratified by octree level (SAMPLE_FRAC of each level, at least
                  MIN_PER_LEVEL), scaled per level by that level's pairs (ratio estimator; its
                  standard error is recorded).
+  c5_flops.json  the serving study's 60 region-of-interest requests (bench.py serve_requests,
+                 seed 5) on its 256^3 lattice (rms 3 spacings, seed 5): per request the tets
+                 selected exactly, the operations on a seeded sample of them sized to about
+                 2e7 candidate cells, scaled by the selected count.
 
-Run from the repository root:  python tests/golden/make_flops.py [c2|c3]
+Run from the repository root:  python tests/golden/make_flops.py [c2|c3|c5]
 """
 
 import json
@@ -150,6 +154,45 @@ def c3():
         json.dump(out, f, indent=1)
 
 
+def c5():
+    sys.path.insert(0, ROOT)
+    from bench import SERVE_N, serve_requests
+    t0 = time.time()
+    lat = make_lattice(SERVE_N, 3.0, seed=5)
+    tets = O.lattice_tets(lat, KUHN)
+    mn, mx = tets.min(1), tets.max(1)
+    del tets
+    ext = np.stack([mx - mn], 0)[0].astype(np.float64)
+    _, rois, dims = serve_requests(60, 5)
+    rng = np.random.default_rng(SAMPLE_SEED)
+    out = {"workload": "serving study: 60 ROI requests (side U[5,50] Mpc/h, 64^3..512^3 cells, seed 5) on the 256^3 "
+                       "lattice at rms 3 spacings (seed 5), Kuhn tets", "requests": []}
+    total = 0.0
+    for i in range(len(dims)):
+        roi = np.asarray(rois[i], np.float64)
+        rf = roi.astype(np.float32)
+        sel = np.flatnonzero(np.all((mn < rf[1::2]) & (rf[0::2] < mx), 1))
+        d = int(dims[i])
+        cell = (roi[1::2] - roi[0::2]) / d
+        per_tet = float(np.mean(np.prod(ext[sel] / cell + 1.0, axis=1))) if sel.size else 0.0
+        k = int(min(sel.size, max(200, 2e7 / max(per_tet, 1.0))))
+        sample = np.sort(rng.choice(sel, k, replace=False)).astype(np.int32) if k else sel.astype(np.int32)
+        f = 0
+        if sample.size:
+            _, st = O.deposit_lattice(lat, roi, d, decomp=KUHN, tet_mass=TET_MASS, tet_ids=sample, precision=2,
+                                      nthreads=os.cpu_count() or 1, stats=True)
+            f = int(st[12])
+        est = f * sel.size / max(sample.size, 1)
+        total += est
+        out["requests"].append({"dims": d, "tets": int(sel.size), "sampled": int(sample.size), "flops": est})
+        print(f"[c5] request {i}: {d}^3, {sel.size} tets, sample {sample.size}, {est:.3e} flops ({time.time() - t0:.0f} s)",
+              flush=True)
+    out["flops"] = total
+    out["counted_by"] = "oracle Counted<double> (add/sub/mul/div), per-request tet sample scaled by the selected tets"
+    with open(os.path.join(HERE, "c5_flops.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "c2"
-    {"c2": c2, "c3": c3}[which]()
+    {"c2": c2, "c3": c3, "c5": c5}[which]()

@@ -58,6 +58,7 @@ ROIS = [([10.0, 40.0, 20.0, 45.0, 30.0, 60.0], 24),    # inside
 def test_requests_match_oracle(lat):
     with A.ResidentLattice.from_lattice(lat) as rl, A.Server(rl, DECOMP, TET_MASS, max_inflight=4) as srv:
         ids = [srv.submit(roi, dims) for roi, dims in ROIS]
+        infos = []
         for (roi, dims), rid in zip(ROIS, ids):
             print(f"[serve] roi={roi} dims={dims}", flush=True)
             got, info = srv.wait(rid, timeout=60)
@@ -65,8 +66,13 @@ def test_requests_match_oracle(lat):
             assert info["n_tets"] == n
             assert rel_err(got, want, 1e-8 * TET_MASS / cell_vol) <= 1e-6 + 1.2e-7
             assert info["t_submit"] <= info["t_admit"] <= info["t_done"]
+            infos.append(info)
         st = srv.stats()
         assert st["requests_done"] == len(ROIS) and st["pending"] == 0 and st["active"] == 0
+        # once the server is idle every step's device time is counted, and the step kernels ran
+        # within the requests' wall time
+        wall = max(i["t_done"] for i in infos) - min(i["t_submit"] for i in infos)
+        assert st["steps"] > 0 and 0.0 < st["step_kernel_s"] <= wall
 
 
 def test_plain_baseline_matches_oracle(lat):
