@@ -339,8 +339,9 @@ __device__ __forceinline__ unsigned separated_cells(const V3<double> t[4], int l
 
 // Clip work class of a partial cell: the axis with the most faces cutting the tet is integrated in
 // closed form (ties -> z); the class counts the remaining x/y faces that cut (from the cell-local
-// AABB) and whether the first of them leaves one sub-tet or a prism.  The class only groups cells
-// of similar clip work; the clip itself decides every face again.
+// AABB) and how many vertices lie inside the first of them.  The class only groups cells of
+// similar clip work (lanes of a pass then take the same branches of the clip tree more often);
+// the clip itself decides every face again.
 // A column segment (force >= 0) keeps its axis: force.
 __device__ __forceinline__ int work_class(const V3<double> s[4], const double mn[3], const double mx[3], int force,
                                           int& axis) {
@@ -366,6 +367,10 @@ __device__ __forceinline__ int work_class(const V3<double> s[4], const double mn
         const bool iz = ge ? s[q].z > 0.0 : s[q].z < 1.0;
         nin += int(fax == 0 ? ix : (fax == 1 ? iy : iz));
     }
+    // classes 1-8: one face; two faces by 1, 2 or 3 vertices inside the first (one sub-tet, a
+    // wedge of three, a corner pair); three and four faces by one vertex inside or more
+    if (nact == 1) return 1;
+    if (nact == 2) return 2 + (nin <= 1 ? 0 : (nin == 2 ? 1 : 2));
     return 2 * nact - 1 + int(nin > 1);
 }
 
