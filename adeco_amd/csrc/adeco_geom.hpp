@@ -475,7 +475,8 @@ struct ZslabSum {
 
 // Leaf of a column segment of ls unit cells along z ([c, c+1], c < ls; ls = 1 is the unit cell):
 // piece q (sign g, |vol6| w) adds g w (1 - below(c) - above(c+1)) to acc[c * kStride] and
-// w (1 - below(c) - above(c+1)) to sab[c * kStride].  The fractions depend on the 4 heights only:
+// w (1 - below(c) - above(c+1)) to sab[c * kStride] (of type S: the magnitudes only set the
+// cancellation-noise threshold, so a kernel may keep them in FP32).  The fractions depend on the 4 heights only:
 // sorted once (z0 <= z1 <= z2 <= z3), a plane h in (z0, z3) has 1, 2 or 3 heights below it, and
 // the side with fewer vertices is computed directly as in beyond_fraction (D is fixed per interval):
 //   (z0, z1]: below = a^3 / D1                 a = h - z0,  D1 = (z1-z0)(z2-z0)(z3-z0)
@@ -483,10 +484,10 @@ struct ZslabSum {
 //             a, b = h - z0, h - z1;  c, e = z2 - h, z3 - h;  D2 = (z2-z0)(z3-z0)(z2-z1)(z3-z1)
 //   (z2, z3): above = e^3 / D3                 D3 = (z3-z0)(z3-z1)(z3-z2)
 // Each plane between two cells is evaluated once for both.
-template <class R, int kStride>
+template <class R, int kStride, class S = double>
 struct ColumnLeaf {
     double* acc;
-    double* sab;
+    S* sab;
     int ls;
     ADECO_HD void operator()(const V3<R> q[4], R g) {
         const R w = absr(vol6(q[0], q[1], q[2], q[3]));
@@ -527,7 +528,7 @@ struct ColumnLeaf {
                 const R m = R(1) - below_prev - above;
                 if (m > R(0)) {
                     acc[(h - 1) * kStride] += double(g * w * m);
-                    sab[(h - 1) * kStride] += double(w * m);
+                    sab[(h - 1) * kStride] += S(w * m);
                 }
             }
             below_prev = below;

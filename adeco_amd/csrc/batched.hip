@@ -79,8 +79,8 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = 64 * kWavesPerBlock;
-constexpr int kBuckets = 10;       // 9 clip work classes + the exact-recompute queue
-constexpr int kExactQueue = 9;
+constexpr int kBuckets = 12;       // 11 clip work classes + the exact-recompute queue
+constexpr int kExactQueue = 11;
 // launches with fewer tiles than this per resident wave drain their queues in merged passes
 constexpr int64_t kMergeDrainTiles = 16;
 constexpr int kQueueCap = 2 * kWaveTile;
@@ -94,7 +94,7 @@ struct WaveLds {
     int2 q[kBuckets][kQueueCap];
     unsigned char qs[kBuckets][kQueueCap];
     double acc[kMaxSeg][kWaveTile];   // signed mass sums (vol6 units) of the column's cells
-    double sab[kMaxSeg][kWaveTile];   // sums of their magnitudes
+    float sab[kMaxSeg][kWaveTile];    // sums of their magnitudes (the noise threshold's scale)
 };
 
 __device__ __forceinline__ int upper_bound_i64(const int64_t* a, int n, int64_t v) {
@@ -367,11 +367,9 @@ __device__ __forceinline__ int work_class(const V3<double> s[4], const double mn
         const bool iz = ge ? s[q].z > 0.0 : s[q].z < 1.0;
         nin += int(fax == 0 ? ix : (fax == 1 ? iy : iz));
     }
-    // classes 1-8: one face; two faces by 1, 2 or 3 vertices inside the first (one sub-tet, a
-    // wedge of three, a corner pair); three and four faces by one vertex inside or more
-    if (nact == 1) return 1;
-    if (nact == 2) return 2 + (nin <= 1 ? 0 : (nin == 2 ? 1 : 2));
-    return 2 * nact - 1 + int(nin > 1);
+    // classes 1-10: one face; two, three and four faces each by 1, 2 or 3 vertices inside the
+    // first (one sub-tet, a wedge of three, a corner pair)
+    return nact == 1 ? 1 : 3 * nact - 4 + (nin <= 1 ? 0 : (nin == 2 ? 1 : 2));
 }
 
 // n / d for n < 2^30 and 1 <= d < 2^30 without the integer division sequence: the FP64 reciprocal
@@ -529,7 +527,7 @@ __device__ __forceinline__ void classify_tile(const TileSource& src, const TileH
 // shows that the tet misses it.  acc/sab: this lane's LDS sums.
 template <int kPrec, class Sources>
 __device__ __forceinline__ unsigned clip_entry(const Sources& srcs, int2 e, int slot, bool exact, double* acc,
-                                               double* sab) {
+                                               float* sab) {
     const TileSource& src = srcs.get(slot);
     const PairData& pd = src.pd[e.x];
     const unsigned u = unsigned(e.y);
@@ -545,7 +543,7 @@ __device__ __forceinline__ unsigned clip_entry(const Sources& srcs, int2 e, int 
     }
     for (int c = 0; c < ls; ++c) {
         acc[c * kWaveTile] = 0.0;
-        sab[c * kWaveTile] = 0.0;
+        sab[c * kWaveTile] = 0.f;
     }
     unsigned redo = 0u, empty = 0u;
     int axis = code;
@@ -569,19 +567,19 @@ __device__ __forceinline__ unsigned clip_entry(const Sources& srcs, int2 e, int 
         // most cells whose signed sum is noise lie outside the tet (pieces that cancel there): a
         // cell that a face plane of the tet separates from it holds nothing
         const unsigned sep = exact ? 0u : separated_cells(s, ls);
-        ColumnLeaf<double, kWaveTile> leaf{acc, sab, ls};
+        ColumnLeaf<double, kWaveTile, float> leaf{acc, sab, ls};
         if (kPrec == ADECO_PREC_FP64) clip_pieces<double, 7>(s, !exact, leaf);
         else clip_pieces<double, 6>(s, false, leaf);
         if (!exact)
             for (int c = 0; c < ls; ++c)
-                redo |= unsigned(signed_is_noisy(acc[c * kWaveTile], sab[c * kWaveTile])) << c;
+                redo |= unsigned(signed_is_noisy(acc[c * kWaveTile], double(sab[c * kWaveTile]))) << c;
         empty = redo & sep;
         redo &= ~empty;
     } else {
         V3<float> s[4];
         cell_local(pd, g, i, j, k, s);
         rotate_axes(s, axis);
-        ColumnLeaf<float, kWaveTile> leaf{acc, sab, ls};
+        ColumnLeaf<float, kWaveTile, float> leaf{acc, sab, ls};
         clip_pieces<float, 7>(s, true, leaf);
         if (kPrec == ADECO_PREC_MIXED) {
             float M = 1.f;
