@@ -20,7 +20,8 @@ leaves on this host's cores, and on one core.
 
 Studies, one JSON line each instead of the headline: --heavy-tail (256^3 lattice with infall
 centres, batched vs plain), --precision (precision modes and near-degenerate tets), --serve (ROI
-requests, continuous batching vs serial plain), --program (the whole program from a snapshot file
+requests, continuous batching vs serial plain, with the step kernel's roofline from the server's
+step-kernel time and tests/golden/c5_flops.json), --program (the whole program from a snapshot file
 to leaf files, SORTED_BY_ID and DARK_SKY, beside the reference program) and --boundary (the drop-in
 deposit_tets called per leaf, plain vs batched form).
 """
