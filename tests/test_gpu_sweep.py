@@ -1,14 +1,22 @@
-"""Seeded random sweep of the tet-soup deposit, plain and batched FP64, against the extended-precision
-oracle: 80 cases over grid sizes 1..64 cells per axis, cubic and non-cubic boxes, both cell scalings,
-and tets from 0.02 to 30 cells across (thin ones down to a tenth of their length in two directions),
-placed in and around the box so that many are cut by its faces.  Per cell
-|gpu - oracle| <= 1e-6 * max(|oracle|, 1e-8 * tet_mass); the grid totals agree to 1e-9.
+"""Seeded random sweeps against the extended-precision oracle, plain and batched FP64.
+
+* Tet soups: 80 cases over grid sizes 1..64 cells per axis, cubic and non-cubic boxes, both cell
+  scalings, and tets from 0.02 to 30 cells across (thin ones down to a tenth of their length in two
+  directions), placed in and around the box so that many are cut by its faces.
+* Octree leaf deposits: 24 lattices (8^3..32^3 particles, rms 0.3-5 spacings, with and without
+  infall centres), octrees split above 300..20000 tets at depths 1-6, 4..32 cells per leaf, all
+  three tet decompositions and both cell scalings; up to 24 leaves each (the 6 heaviest and a seeded
+  spread), the oracle given every tet whose AABB meets the leaf box (its own selection, not the
+  library's binning).
+Per cell |gpu - oracle| <= 1e-6 * max(|oracle|, 1e-8 * tet_mass); grid totals (soups) and the sum
+over all leaves against the lattice's mass inside the root (exact scaling) agree to 1e-9.
 """
 
 import numpy as np
 import pytest
 
 import adeco_amd as A
+from adeco_amd.lattice import make_lattice
 from tests import _oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -76,3 +84,59 @@ def test_sweep_covers_its_regimes():
     # a tet's extent in cells (its first tet; a case's tets share the scale)
     sizes = [np.ptp(c[0][0], axis=0).max() / ((c[1][1::2] - c[1][0::2]) / c[2]).mean() for c in cases]
     assert min(sizes) < 0.1 and max(sizes) > 10
+
+
+LEAF_CASES = 24
+
+
+def make_leaf_case(seed):
+    """(lattice n, rms, infall centres, split threshold, depth, cells per leaf, decomposition, scaling)."""
+    rng = np.random.default_rng(5000 + seed)
+    n = int(rng.integers(8, 33))
+    rms = float(rng.uniform(0.3, 5.0))
+    infall = int(rng.choice([0, 0, 4, 12]))
+    max_tets = int(rng.choice([300, 1000, 3000, 20000]))
+    depth = int(rng.integers(2, 7))
+    dims = int(rng.choice([4, 8, 12, 16, 24, 32]))
+    # a tet spans at most ~6 cells of its deepest leaves per axis (dims 2^depth / n): the plain form
+    # walks a tet's cells on one thread and the oracle clips every one of them on the CPU
+    while dims > 4 and dims * 2 ** depth / n > 6:
+        dims //= 2
+    while depth > 1 and dims * 2 ** depth / n > 6:
+        depth -= 1
+    decomp = int(rng.choice([A.DECOMP_REF_EFFECTIVE, A.DECOMP_REF_INTENDED, A.DECOMP_KUHN]))
+    scaling = O.EXACT if rng.random() < 0.7 else O.REFERENCE
+    return n, rms, infall, max_tets, depth, dims, decomp, scaling, rng
+
+
+@pytest.mark.parametrize("block", range(3))
+def test_random_octree_leaves_match_oracle(block):
+    for seed in range(block * LEAF_CASES // 3, (block + 1) * LEAF_CASES // 3):
+        n, rms, infall, max_tets, depth, dims, decomp, scaling, rng = make_leaf_case(seed)
+        print(f"[sweep] leaves seed {seed}: {n}^3 rms {rms:.1f} infall {infall}, split {max_tets} depth {depth}, "
+              f"{dims}^3 cells, decomposition {decomp}, scaling {scaling}", flush=True)
+        lat = make_lattice(n, rms, seed=seed + 100, infall_centres=infall)
+        with O.cell_scaling(scaling), A.ResidentLattice.from_lattice(lat) as rl, A.Octree(rl, max_tets, depth) as tree:
+            counts = tree.bin(decomp)
+            nl = tree.num_leaves
+            boxes = tree.leaf_boxes_f64(exact=(scaling == O.EXACT))
+            pick = np.unique(np.concatenate([np.argsort(-counts)[:6], rng.choice(nl, min(nl, 18), replace=False)]))
+            tets = O.lattice_tets(lat, decomp)
+            mn, mx = tets.min(1).astype(np.float64), tets.max(1).astype(np.float64)
+            want = []
+            for l in pick:
+                b = boxes[l]
+                pad = 1e-6 * (b[1::2] - b[0::2])
+                ids = np.flatnonzero(np.all((mn < b[1::2] + pad) & (b[0::2] - pad < mx), 1)).astype(np.int32)
+                want.append(O.deposit_lattice(lat, b, dims, decomp=decomp, tet_mass=TET_MASS, tet_ids=ids, precision=3,
+                                              nthreads=16))
+            want = np.array(want)
+            inside = None
+            if scaling == O.EXACT:
+                inside = O.deposit_lattice(lat, tree.root_box_f64(), 1, decomp=decomp, tet_mass=TET_MASS).sum()
+            for name, mode in (("plain", A.MODE_PLAIN), ("batched", A.MODE_BATCHED)):
+                tree.deposit(dims, decomposition=decomp, tet_mass=TET_MASS, mode=mode, precision=A.PREC_FP64)
+                g = tree.leaf_grids()
+                assert O.max_rel_err(g[pick], want, tet_mass=TET_MASS) <= 1e-6, (seed, name)
+                if inside is not None:
+                    assert abs(g.sum() - inside) <= 1e-9 * inside, (seed, name)
