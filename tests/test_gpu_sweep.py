@@ -8,6 +8,8 @@
   three tet decompositions and both cell scalings; up to 24 leaves each (the 6 heaviest and a seeded
   spread), the oracle given every tet whose AABB meets the leaf box (its own selection, not the
   library's binning).
+* ROI requests: 30 random boxes on three random lattices, submitted together to servers with 1-8
+  slots (test_random_requests_match_oracle).
 Per cell |gpu - oracle| <= 1e-6 * max(|oracle|, 1e-8 * tet_mass); grid totals (soups) and the sum
 over all leaves against the lattice's mass inside the root (exact scaling) agree to 1e-9.
 """
@@ -140,3 +142,53 @@ def test_random_octree_leaves_match_oracle(block):
                 assert O.max_rel_err(g[pick], want, tet_mass=TET_MASS) <= 1e-6, (seed, name)
                 if inside is not None:
                     assert abs(g.sum() - inside) <= 1e-9 * inside, (seed, name)
+
+
+SERVE_BLOCKS = 3
+SERVE_REQUESTS = 10
+
+
+@pytest.mark.parametrize("block", range(SERVE_BLOCKS))
+def test_random_requests_match_oracle(block):
+    """Seeded random ROI requests, submitted together to a server with a random number of slots, on a
+    random lattice: boxes 1-60 Mpc/h per side (non-cubic, some partly or wholly outside the
+    lattice), 4..64 cells per axis (at most ~8 cells across a tet), FP32 density vs the oracle's
+    deposit of the tets whose float AABB strictly overlaps the box (its own selection), per cell to
+    1e-6 plus one FP32 rounding, and the selected tet count exactly."""
+    rng = np.random.default_rng(9000 + block)
+    n = int(rng.integers(10, 25))
+    lat = make_lattice(n, float(rng.uniform(0.5, 4.0)), seed=200 + block, infall_centres=int(rng.choice([0, 6])))
+    tets = O.lattice_tets(lat, A.DECOMP_KUHN)
+    mn, mx = tets.min(1), tets.max(1)
+    tet_size = float(np.median(np.max(mx - mn, axis=1)))
+    reqs = []
+    for _ in range(SERVE_REQUESTS):
+        side = np.exp(rng.uniform(np.log(1.0), np.log(60.0), 3))
+        if rng.random() < 0.5:
+            side[:] = side[0]
+        lo = rng.uniform(-10.0, 105.0, 3) - side / 2
+        roi = np.empty(6)
+        roi[0::2] = lo
+        roi[1::2] = lo + side
+        cap = max(4, int(8 * float(np.min(side)) / tet_size))
+        dims = int(min(int(rng.choice([4, 8, 13, 16, 24, 32, 48, 64])), cap))
+        reqs.append((roi, dims))
+    slots = int(rng.integers(1, 9))
+    with O.cell_scaling(O.EXACT), A.ResidentLattice.from_lattice(lat) as rl, \
+            A.Server(rl, A.DECOMP_KUHN, TET_MASS, max_inflight=slots) as srv:
+        ids = [srv.submit(roi, dims) for roi, dims in reqs]
+        for (roi, dims), rid in zip(reqs, ids):
+            print(f"[sweep] serve block {block}: {n}^3, {slots} slots, box {np.round(roi, 2)}, {dims}^3", flush=True)
+            got, info = srv.wait(rid, timeout=120)
+            rf = roi.astype(np.float32)
+            sel = np.flatnonzero(np.all((mn < rf[1::2]) & (rf[0::2] < mx), 1)).astype(np.int32)
+            assert info["n_tets"] == sel.size
+            cell_vol = float(np.prod((roi[1::2] - roi[0::2]) / dims))
+            if sel.size:
+                want = O.deposit_lattice(lat, roi, dims, decomp=A.DECOMP_KUHN, tet_mass=TET_MASS, tet_ids=sel,
+                                         precision=3, nthreads=16) / cell_vol
+            else:
+                want = np.zeros(dims ** 3)
+            g = np.asarray(got, np.float64).ravel()
+            floor = 1e-8 * TET_MASS / cell_vol
+            assert float(np.max(np.abs(g - want) / np.maximum(np.abs(want), floor))) <= 1e-6 + 1.2e-7
