@@ -18,17 +18,17 @@
 //                       tet's 4 face planes at the extreme corners (a relative margin of 1e-12: an
 //                       uncertain segment is clipped, never skipped or filled).  Segments inside the
 //                       tet and cells holding the whole tet are deposited at once; partial ones go
-//                       to one of 9 per-wave LDS queues keyed by their clip work class (number of
+//                       to one of 22 per-wave LDS queues keyed by their clip work class (number of
 //                       x/y faces that cut the tet once the column axis - for a single cell the axis
-//                       with the most cutting faces - is taken as the closed-form z axis, and
-//                       whether the first cut leaves 1 or 3 sub-tets).  A queue that reaches 64
-//                       entries is clipped by the whole wave, so the lanes of one clip follow nearly
-//                       the same path through the nested clip.
+//                       with the most cutting faces - is taken as the closed-form z axis, and how
+//                       many vertices lie inside each of the clip tree's first two faces).  A queue
+//                       that reaches 64 entries is clipped by the whole wave, so the lanes of one
+//                       clip follow nearly the same path through the nested clip.
 // One x/y clip serves all cells of a segment: every leaf piece is integrated over the z planes
 // between the cells in closed form, each plane once for the two cells it separates, into per-lane
 // LDS sums.  The clip uses the signed tree (a cut leaving 3 vertices inside gives the tet minus the
 // corner beyond the plane: fewer pieces); a cell whose sum is below 1e-6 of the magnitudes it was
-// summed from may be cancellation noise, so it goes alone to a tenth per-wave queue whose waves
+// summed from may be cancellation noise, so it goes alone to one more per-wave queue whose waves
 // clip it with the exact tree (non-overlapping pieces, all terms positive).  Masses go to the grids
 // with FP64 global atomics.  Precision modes for the clip:
 //   FP64  all FP64;
@@ -79,23 +79,32 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = 64 * kWavesPerBlock;
-constexpr int kBuckets = 12;       // 11 clip work classes + the exact-recompute queue
-constexpr int kExactQueue = 11;
+constexpr int kBuckets = 23;       // 22 clip work classes + the exact-recompute queue
+constexpr int kExactQueue = 22;
 // launches with fewer tiles than this per resident wave drain their queues in merged passes
 constexpr int64_t kMergeDrainTiles = 16;
-constexpr int kQueueCap = 2 * kWaveTile;
 
-// Per-wavefront LDS: pair-start markers of the current tile, the bucketed partial-candidate queues
-// and each lane's per-cell sums of the column it clips.  A queue entry is (pair, first cell |
-// closed-form axis << 30) plus the candidate space it belongs to.
+// Per-wavefront LDS: the bucketed partial-candidate queues (64 entries each), the entries a tile
+// could not place because their queue was full (at most one per lane, written back once that queue
+// has been clipped), and each lane's per-cell sums of the column it clips; the pair-start markers
+// of the tile being classified share the sums' storage (classification and clipping alternate).
+// A queue entry is (pair, first cell | closed-form axis << 30) plus the candidate space it belongs
+// to.  qn[b] counts queue b's entries plus its carried ones.
 struct WaveLds {
-    int mark[kWaveTile];
-    int qn[16];
-    int2 q[kBuckets][kQueueCap];
-    unsigned char qs[kBuckets][kQueueCap];
-    double acc[kMaxSeg][kWaveTile];   // signed mass sums (vol6 units) of the column's cells
+    int qn[24];
+    int2 q[kBuckets][kWaveTile];
+    unsigned char qs[kBuckets][kWaveTile];
+    int2 carry[kWaveTile];            // per lane: its carried entry,
+    unsigned char carry_slot[kWaveTile];
+    signed char carry_key[kWaveTile]; // its queue (-1: none)
+    unsigned char carry_pos[kWaveTile];  // and its position once that queue has been clipped
+    union {
+        double acc[kMaxSeg][kWaveTile];   // signed mass sums (vol6 units) of the column's cells
+        int mark[kWaveTile];
+    };
     float sab[kMaxSeg][kWaveTile];    // sums of their magnitudes (the noise threshold's scale)
 };
+static_assert(kBuckets <= 24 && kBuckets < 32, "queue counts and the drain's prefix sum cover 24 lanes");
 
 __device__ __forceinline__ int upper_bound_i64(const int64_t* a, int n, int64_t v) {
     int lo = 0, hi = n;
@@ -339,9 +348,10 @@ __device__ __forceinline__ unsigned separated_cells(const V3<double> t[4], int l
 
 // Clip work class of a partial cell: the axis with the most faces cutting the tet is integrated in
 // closed form (ties -> z); the class counts the remaining x/y faces that cut (from the cell-local
-// AABB) and how many vertices lie inside the first of them.  The class only groups cells of
-// similar clip work (lanes of a pass then take the same branches of the clip tree more often);
-// the clip itself decides every face again.
+// AABB), whether one face of each axis cuts or both of one, and how many vertices lie inside the
+// first two faces of the clip tree.  The class only groups cells of similar clip work (lanes of a
+// pass then take the same branches of the clip tree more often); the clip itself decides every
+// face again.
 // A column segment (force >= 0) keeps its axis: force.
 __device__ __forceinline__ int work_class(const V3<double> s[4], const double mn[3], const double mx[3], int force,
                                           int& axis) {
@@ -352,24 +362,37 @@ __device__ __forceinline__ int work_class(const V3<double> s[4], const double mn
     // the clip's x and y after rotation: (y, z) for axis 0, (z, x) for axis 1, (x, y) for axis 2
     const bool lo0 = axis == 0 ? ly : (axis == 1 ? lz : lx), hi0 = axis == 0 ? hy : (axis == 1 ? hz : hx);
     const bool lo1 = axis == 0 ? lz : (axis == 1 ? lx : ly), hi1 = axis == 0 ? hz : (axis == 1 ? hx : hy);
-    const int nact = int(lo0) + int(hi0) + int(lo1) + int(hi1);
+    const int nx = int(lo0) + int(hi0), ny = int(lo1) + int(hi1);
+    const int nact = nx + ny;
     if (nact == 0) return 0;
-    // first cutting face in the clip's order x >= 0, x <= 1, y >= 0, y <= 1
-    const bool on0 = lo0 || hi0;
-    const int fax = on0 ? (axis == 0 ? 1 : (axis == 1 ? 2 : 0)) : (axis == 0 ? 2 : (axis == 1 ? 0 : 1));
-    const bool ge = on0 ? lo0 : lo1;
+    if (nact == 1) return 1;
+    // the first two faces of the clip tree (clip_pieces' canonical frame): X is the clip axis with
+    // more cutting faces (ties -> x), Y the other; first X >= 0 (X <= 1 when that is X's only
+    // cutting face), then Y's cutting face when each axis has one, else X <= 1
+    const bool swap = ny > nx;
+    // the clip's x and y as axes of the cell-local frame (see rotate_axes)
+    const int xp = axis == 0 ? 1 : (axis == 1 ? 2 : 0), yp = axis == 0 ? 2 : (axis == 1 ? 0 : 1);
+    const int xax = swap ? yp : xp, yax = swap ? xp : yp;
+    const bool xlo = swap ? lo1 : lo0, ylo = swap ? lo0 : lo1;
+    const bool xy = nx == 1 && ny == 1;
+    const int ax1 = xax, ax2 = xy ? yax : xax;
+    const bool ge1 = xlo, ge2 = xy && ylo;
     // select comparison results, not coordinates: a select of coordinates becomes a select of
     // addresses into a stack copy of the tet
-    int nin = 0;
+    int n1 = 0, n2 = 0;
     for (int q = 0; q < 4; ++q) {
-        const bool ix = ge ? s[q].x > 0.0 : s[q].x < 1.0;
-        const bool iy = ge ? s[q].y > 0.0 : s[q].y < 1.0;
-        const bool iz = ge ? s[q].z > 0.0 : s[q].z < 1.0;
-        nin += int(fax == 0 ? ix : (fax == 1 ? iy : iz));
+        const bool gx = s[q].x > 0.0, gy = s[q].y > 0.0, gz = s[q].z > 0.0;
+        const bool bx = s[q].x < 1.0, by = s[q].y < 1.0, bz = s[q].z < 1.0;
+        n1 += int(ax1 == 0 ? (ge1 ? gx : bx) : (ax1 == 1 ? (ge1 ? gy : by) : (ge1 ? gz : bz)));
+        n2 += int(ax2 == 0 ? (ge2 ? gx : bx) : (ax2 == 1 ? (ge2 ? gy : by) : (ge2 ? gz : bz)));
     }
-    // classes 1-10: one face; two, three and four faces each by 1, 2 or 3 vertices inside the
-    // first (one sub-tet, a wedge of three, a corner pair)
-    return nact == 1 ? 1 : 3 * nact - 4 + (nin <= 1 ? 0 : (nin == 2 ? 1 : 2));
+    // vertices inside the first and second face (1-3 each; with both X faces cutting, every vertex
+    // is inside one of them, so n1 + n2 >= 4)
+    const int t1 = n1 <= 1 ? 0 : (n1 == 2 ? 1 : 2), t2 = n2 <= 1 ? 0 : (n2 == 2 ? 1 : 2);
+    if (xy) return 2 + 3 * t1 + t2;                                           // 2-10
+    if (nact == 2) return 11;                                                 // both X faces
+    if (nact == 3) return 12 + (t1 == 0 ? 0 : (t1 == 1 ? (t2 <= 1 ? 1 : 2) : 3 + t2));  // 12-17
+    return 18 + (t1 <= 1 ? 0 : 1 + t2);                                       // 18-21
 }
 
 // n / d for n < 2^30 and 1 <= d < 2^30 without the integer division sequence: the FP64 reciprocal
@@ -509,12 +532,22 @@ __device__ __forceinline__ void classify_tile(const TileSource& src, const TileH
             }
         }
     }
-    // append the partial cells to their queues (every queue holds < 64 here, so < 128 after)
+    // append the partial cells to their queues.  Every queue holds < 64 entries here, so a count
+    // reaches at most 127; an entry beyond 64 is carried (position - 64 once its queue is clipped).
+    int ckey = -1;
     if (key >= 0) {
         const int pos = atomicAdd(&L.qn[key], 1);
-        L.q[key][pos] = make_int2(p, cell);
-        L.qs[key][pos] = (unsigned char)slot;
+        if (pos < kWaveTile) {
+            L.q[key][pos] = make_int2(p, cell);
+            L.qs[key][pos] = (unsigned char)slot;
+        } else {
+            ckey = key;
+            L.carry[lane] = make_int2(p, cell);
+            L.carry_slot[lane] = (unsigned char)slot;
+            L.carry_pos[lane] = (unsigned char)(pos - kWaveTile);
+        }
     }
+    L.carry_key[lane] = (signed char)ckey;
     n_partial += __popcll(__ballot(key >= 0));
     wave_sync();
 }
@@ -621,7 +654,8 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
     __shared__ WaveLds lds[kWavesPerBlock];
     const int lane = threadIdx.x & 63;
     WaveLds& L = lds[threadIdx.x >> 6];
-    if (lane < 16) L.qn[lane] = 0;
+    if (lane < 24) L.qn[lane] = 0;
+    L.carry_key[lane] = -1;
     wave_sync();
     unsigned long long n_partial = 0, fb = 0;
     int64_t it = 0, it_end = 0;
@@ -634,20 +668,26 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
     int2 pe = make_int2(0, 0);
     int pslot = 0;
     for (;;) {
-        // pending cells go to the exact queue one per lane and round while it holds < 64; at 64 it
-        // is clipped first (exact clips leave no cells behind), so it never holds more than 127
-        if (__ballot(pend != 0u) && L.qn[kExactQueue] < kWaveTile) {
-            if (pend) {
+        // pending cells go to the exact queue, one per lane and round, up to its 64 entries; a full
+        // exact queue is clipped first (exact clips leave no cells behind)
+        const unsigned long long pm = __ballot(pend != 0u);
+        const int nexact = pm ? __builtin_amdgcn_readfirstlane(L.qn[kExactQueue]) : kWaveTile;
+        if (nexact < kWaveTile) {
+            const int room = kWaveTile - nexact;
+            const int rank =
+                int(__builtin_amdgcn_mbcnt_hi(unsigned(pm >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(pm), 0u)));
+            if (pend && rank < room) {
                 const int c = __builtin_ctz(pend);
                 pend &= pend - 1u;
                 const unsigned u = unsigned(pe.y);
                 const int ax = int(u >> 30);
                 const unsigned cell = (u & 0x3fffffffu) + (unsigned(c) << (10 * ax));
-                const int pos = atomicAdd(&L.qn[kExactQueue], 1);
-                L.q[kExactQueue][pos] = make_int2(pe.x, int(cell | (3u << 30)));
-                L.qs[kExactQueue][pos] = (unsigned char)pslot;
+                L.q[kExactQueue][nexact + rank] = make_int2(pe.x, int(cell | (3u << 30)));
+                L.qs[kExactQueue][nexact + rank] = (unsigned char)pslot;
                 ++fb;
             }
+            wave_sync();
+            if (lane == 0) L.qn[kExactQueue] = nexact + min(int(__popcll(pm)), room);
             wave_sync();
             continue;
         }
@@ -669,7 +709,7 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
             if (kMergeDrain && draining && b != kExactQueue && cnt < kWaveTile) {
                 const int qc = lane < kExactQueue ? qv : 0;
                 int pre = qc;
-                for (int d = 1; d < 16; d <<= 1) {
+                for (int d = 1; d < 32; d <<= 1) {
                     const int o = __shfl_up(pre, d, 64);
                     if (lane >= d) pre += o;
                 }
@@ -686,15 +726,23 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
                 wave_sync();
                 if (lane < kExactQueue) L.qn[lane] = qc - tq;
             } else {
+                // the queue's entries; what the count holds beyond 64 are carried entries, which
+                // take the freed positions
                 const int take = min(cnt, kWaveTile);
                 const int rest = cnt - take;
                 if (lane < take) {
-                    e = L.q[b][rest + lane];
-                    slot = L.qs[b][rest + lane];
+                    e = L.q[b][lane];
+                    slot = L.qs[b][lane];
                     have = true;
                 }
                 wave_sync();
                 if (lane == 0) L.qn[b] = rest;
+                if (rest > 0 && L.carry_key[lane] == b) {
+                    const int at = L.carry_pos[lane];
+                    L.q[b][at] = L.carry[lane];
+                    L.qs[b][at] = L.carry_slot[lane];
+                    L.carry_key[lane] = -1;
+                }
             }
             wave_sync();
             // (a pass over another queue starts only with no cells pending; exact clips leave none)
