@@ -83,6 +83,10 @@ constexpr int kBuckets = 23;       // 22 clip work classes + the exact-recompute
 constexpr int kExactQueue = 22;
 // launches with fewer tiles than this per resident wave drain their queues in merged passes
 constexpr int64_t kMergeDrainTiles = 16;
+// how a wave drains its queues once the work items run out: a pass per queue (a large launch,
+// where the drain is a small part of the work), passes filled from the classes of one group (a
+// step of the request server, which drains every step), or from all the classes (a small launch)
+constexpr int kDrainPerClass = 0, kDrainGroup = 1, kDrainAll = 2;
 
 // Per-wavefront LDS: the bucketed partial-candidate queues (64 entries each), the entries a tile
 // could not place because their queue was full (at most one per lane, written back once that queue
@@ -395,6 +399,13 @@ __device__ __forceinline__ int work_class(const V3<double> s[4], const double mn
     return 18 + (t1 <= 1 ? 0 : 1 + t2);                                       // 18-21
 }
 
+// Work classes whose clips share the tree's shape (no or one cutting face; a corner of the
+// column's cross-section; both faces of one axis; three faces; four faces): a drain pass fills its
+// lanes from the classes of one group.
+__device__ __forceinline__ int class_group(int c) {
+    return c <= 1 ? 0 : (c <= 10 ? 1 : (c == 11 ? 2 : (c <= 17 ? 3 : 4)));
+}
+
 // n / d for n < 2^30 and 1 <= d < 2^30 without the integer division sequence: the FP64 reciprocal
 // refined once is accurate to ~2^-50, so the truncated product is the quotient or one less (exact
 // multiples), fixed by one integer correction each way
@@ -647,7 +658,7 @@ struct ManySources {
 // directly (one source) or through `entries`.  The wave pops `chunk` items per work-queue access;
 // a queue reaching 64 entries is clipped at once; when the items run out the queues are drained.
 // Every wave leaves the loop once the counter passes n_items and its queues are empty.
-template <int kPrec, bool kMergeDrain, class Sources>
+template <int kPrec, int kDrain, class Sources>
 __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __restrict__ entries, int64_t n_items,
                                           int chunk, unsigned* __restrict__ queue_counter,
                                           unsigned long long* __restrict__ counters) {
@@ -699,22 +710,24 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
         if (ready) {
             const int b = (ready >> kExactQueue) & 1ull ? kExactQueue : __builtin_ctzll(ready);
             const int cnt = __builtin_amdgcn_readfirstlane(__shfl(qv, b, 64));
-            // this pass's entry of each lane: from queue b, or when draining a small launch with no
-            // full queue, up to 64 entries across all the work classes (they only group similar
-            // work; the clip does not depend on them), from the top of each queue, lowest class
-            // first.  Either way a pass takes >= 1 entry, so the drain ends.
+            // this pass's entry of each lane: from queue b, or when a server step or a small launch
+            // drains with no full queue, up to 64 entries across the work classes of b's group (of
+            // all groups in a small launch; the classes only group similar work, the clip does not
+            // depend on them), from the top of each queue, lowest class first.  Either way a pass
+            // takes >= 1 entry, so the drain ends.
             int2 e = make_int2(0, 0);
             int slot = 0;
             bool have = false;
-            if (kMergeDrain && draining && b != kExactQueue && cnt < kWaveTile) {
-                const int qc = lane < kExactQueue ? qv : 0;
+            if (kDrain != kDrainPerClass && draining && b != kExactQueue && cnt < kWaveTile) {
+                const bool merge = lane < kExactQueue && (kDrain == kDrainAll || class_group(lane) == class_group(b));
+                const int qc = merge ? qv : 0;
                 int pre = qc;
                 for (int d = 1; d < 32; d <<= 1) {
                     const int o = __shfl_up(pre, d, 64);
                     if (lane >= d) pre += o;
                 }
                 const int excl = pre - qc;
-                const int tq = lane < kExactQueue ? max(0, min(qc, kWaveTile - excl)) : 0;
+                const int tq = merge ? max(0, min(qc, kWaveTile - excl)) : 0;
                 for (int qi = 0; qi < kExactQueue; ++qi) {
                     const int ex = __shfl(excl, qi, 64), tk = __shfl(tq, qi, 64), cq = __shfl(qc, qi, 64);
                     if (lane >= ex && lane < ex + tk) {
@@ -724,7 +737,7 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
                     }
                 }
                 wave_sync();
-                if (lane < kExactQueue) L.qn[lane] = qc - tq;
+                if (merge) L.qn[lane] = qc - tq;
             } else {
                 // the queue's entries; what the count holds beyond 64 are carried entries, which
                 // take the freed positions
@@ -787,7 +800,8 @@ __global__ void __launch_bounds__(kBlock, 2) k_deposit_waves(TileSource src, int
         src.n_cand = src.dev_sizes[0];
         src.n_tiles = src.dev_sizes[1];
     }
-    wave_loop<kPrec, kMergeDrain>(OneSource{src}, nullptr, src.n_tiles, chunk, queue_counter, counters);
+    wave_loop<kPrec, kMergeDrain ? kDrainAll : kDrainPerClass>(OneSource{src}, nullptr, src.n_tiles, chunk,
+                                                                queue_counter, counters);
 }
 
 template <int kPrec, bool kMergeDrain>
@@ -795,7 +809,8 @@ __global__ void __launch_bounds__(kBlock, 2) k_deposit_step(const TileSource* __
                                                          const int2* __restrict__ entries, int n_entries, int chunk,
                                                          unsigned* __restrict__ queue_counter,
                                                          unsigned long long* __restrict__ counters) {
-    wave_loop<kPrec, kMergeDrain>(ManySources{sources}, entries, n_entries, chunk, queue_counter, counters);
+    wave_loop<kPrec, kMergeDrain ? kDrainAll : kDrainGroup>(ManySources{sources}, entries, n_entries, chunk,
+                                                             queue_counter, counters);
 }
 
 __global__ void k_zero_words(unsigned* p, int n) {
