@@ -64,7 +64,7 @@ HBM_PEAK_GBS = 8000.0
 # and WRITE_SIZE (KiB per dispatch) scaled by what each reads for a 1 GiB copy (2.0 and 1.0),
 # profiles/deposit_waves_c3_pmc.md (256^3 infall workload) and profiles/deposit_waves_c2_pmc.md
 # (128^3)
-TRAFFIC_BYTES = {"c3": 15198597.375 * 2 * 1024 + 17971139.40625 * 1024, "c2": 1732849.6875 * 2 * 1024 + 1916026.5 * 1024}
+TRAFFIC_BYTES = {"c3": 22706736.9375 * 2 * 1024 + 18270762.125 * 1024, "c2": 2575602.0 * 2 * 1024 + 1951831.875 * 1024}
 
 
 def log(*a):
