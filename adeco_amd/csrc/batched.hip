@@ -220,66 +220,134 @@ __global__ void __launch_bounds__(256) k_pair_grid(const int64_t* __restrict__ g
     for (int64_t i = grid_off[g] + threadIdx.x; i < end; i += blockDim.x) pair_grid[i] = g;
 }
 
-// exclusive scan of uint32 counts into int64 offsets: block-local pass, block-sum pass, add pass
+// Exclusive scan of uint32 counts into int64 offsets out[0..n] (out[n] = the total), by
+// reduce-then-scan: the counts' sums per block of kScanBlock (k_scan_reduce), one workgroup
+// scanning those sums (k_scan_sums), then each block's local scan offset by its block's prefix
+// (k_scan_apply): the counts are read twice and the offsets written once.  A lane takes 4
+// consecutive counts; a block's 4 waves scan by shuffles and one exchange of wave totals.
+// (n_dev: the count on the device, capped at n; blocks past it sum to zero and write nothing, except
+// the one holding out[n_dev].)
 constexpr int kScanBlock = 1024;
 
-// (n_dev: the count on the device, capped at n; blocks beyond it contribute zero)
-__global__ void __launch_bounds__(256) k_scan_local(const unsigned* __restrict__ in, int64_t n,
-                                                    const int64_t* __restrict__ n_dev, int64_t* __restrict__ out,
-                                                    int64_t* __restrict__ block_sum) {
-    __shared__ int64_t s[kScanBlock];
+__device__ __forceinline__ int64_t shfl_up_i64(int64_t v, int d) {
+    const int lo = __shfl_up(int(uint64_t(v) & 0xffffffffu), d, 64);
+    const int hi = __shfl_up(int(uint64_t(v) >> 32), d, 64);
+    return int64_t((uint64_t(unsigned(hi)) << 32) | unsigned(lo));
+}
+
+__device__ __forceinline__ int64_t shfl_xor_i64(int64_t v, int m) {
+    const int lo = __shfl_xor(int(uint64_t(v) & 0xffffffffu), m, 64);
+    const int hi = __shfl_xor(int(uint64_t(v) >> 32), m, 64);
+    return int64_t((uint64_t(unsigned(hi)) << 32) | unsigned(lo));
+}
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const unsigned* __restrict__ in, int64_t n,
+                                                     const int64_t* __restrict__ n_dev, int64_t* __restrict__ block_sum) {
+    __shared__ int64_t wsum[4];
     if (n_dev) n = min(*n_dev, n);
     const int64_t base = int64_t(blockIdx.x) * kScanBlock;
+    const int t = int(threadIdx.x);
     if (base >= n) {
-        if (threadIdx.x == 0) block_sum[blockIdx.x] = 0;
+        if (t == 0) block_sum[blockIdx.x] = 0;
         return;
     }
-    for (int i = threadIdx.x; i < kScanBlock; i += blockDim.x) s[i] = (base + i < n) ? int64_t(in[base + i]) : 0;
+    const int64_t i0 = base + 4 * t;
+    int64_t v = 0;
+    for (int k = 0; k < 4; ++k) v += i0 + k < n ? int64_t(in[i0 + k]) : 0;
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_i64(v, m);
+    if ((t & 63) == 0) wsum[t >> 6] = v;
     __syncthreads();
-    for (int off = 1; off < kScanBlock; off <<= 1) {
-        int64_t add[kScanBlock / 256];
-        for (int k = 0; k < kScanBlock / 256; ++k) {
-            const int i = threadIdx.x + 256 * k;
-            add[k] = i >= off ? s[i - off] : 0;
-        }
-        __syncthreads();
-        for (int k = 0; k < kScanBlock / 256; ++k) s[threadIdx.x + 256 * k] += add[k];
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < kScanBlock; i += blockDim.x)
-        if (base + i < n) out[base + i] = s[i] - int64_t(in[base + i]);
-    if (threadIdx.x == 0) block_sum[blockIdx.x] = s[kScanBlock - 1];
+    if (t == 0) block_sum[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-// one workgroup of 256 lanes: exclusive scan of the block sums in chunks of 256 with a carry
-__global__ void __launch_bounds__(256) k_scan_sums(int64_t* __restrict__ bs, int64_t nb, int64_t* __restrict__ total) {
-    __shared__ int64_t s[256];
-    int64_t carry = 0;
-    // (blocks past a device-side count hold zero, so scanning all nb of them is the same)
-    for (int64_t base = 0; base < nb; base += 256) {
-        const int64_t i = base + threadIdx.x;
-        const int64_t v = i < nb ? bs[i] : 0;
-        s[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            const int64_t a = int(threadIdx.x) >= off ? s[threadIdx.x - off] : 0;
-            __syncthreads();
-            s[threadIdx.x] += a;
-            __syncthreads();
-        }
-        if (i < nb) bs[i] = carry + s[threadIdx.x] - v;
-        carry += s[255];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *total = carry;
-}
-
-__global__ void __launch_bounds__(256) k_scan_add(int64_t* __restrict__ out, int64_t n, const int64_t* __restrict__ n_dev,
-                                                  const int64_t* __restrict__ bs, const int64_t* __restrict__ total) {
+// (launched with n / kScanBlock + 1 blocks, so that the block holding out[n] exists)
+__global__ void __launch_bounds__(256) k_scan_apply(const unsigned* __restrict__ in, int64_t n,
+                                                    const int64_t* __restrict__ n_dev,
+                                                    const int64_t* __restrict__ block_sum,
+                                                    const int64_t* __restrict__ total, int64_t* __restrict__ out) {
+    __shared__ int64_t wsum[4];
     if (n_dev) n = min(*n_dev, n);
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) out[i] += bs[i / kScanBlock];
-    if (i == n) out[n] = *total;
+    const int64_t base = int64_t(blockIdx.x) * kScanBlock;
+    const int t = int(threadIdx.x), lane = t & 63, w = t >> 6;
+    if (base >= n) {
+        if (base == n && t == 0) out[n] = *total;
+        return;
+    }
+    const int64_t i0 = base + 4 * t;
+    int64_t c[4];
+    for (int k = 0; k < 4; ++k) c[k] = i0 + k < n ? int64_t(in[i0 + k]) : 0;
+    const int64_t own = c[0] + c[1] + c[2] + c[3];
+    int64_t v = own;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t o = shfl_up_i64(v, d);
+        if (lane >= d) v += o;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    int64_t run = block_sum[blockIdx.x] + v - own;
+    for (int k = 0; k < w; ++k) run += wsum[k];
+    for (int k = 0; k < 4; ++k) {
+        const int64_t i = i0 + k;
+        if (i < n) out[i] = run;
+        else if (i == n) out[n] = *total;
+        run += c[k];
+    }
+}
+
+// one workgroup of 1024 lanes: exclusive scan of the nb block sums.  Each lane owns a contiguous
+// segment of ceil(nb / 1024) sums: it adds them up, the lanes' totals are scanned across the
+// workgroup (wave shuffles, then the 16 wave totals), and each lane writes its segment's
+// exclusive prefixes from its lane offset.  (Blocks past a device-side count hold zero, so
+// scanning all nb of them is the same.)
+constexpr int kScanSumsBlock = 1024;
+
+__global__ void __launch_bounds__(kScanSumsBlock) k_scan_sums(int64_t* __restrict__ bs, int64_t nb,
+                                                              int64_t* __restrict__ total) {
+    __shared__ int64_t wsum[kScanSumsBlock / 64];
+    const int t = int(threadIdx.x), lane = t & 63, w = t >> 6;
+    const int64_t per = (nb + kScanSumsBlock - 1) / kScanSumsBlock;
+    const int64_t lo = min(nb, int64_t(t) * per), hi = min(nb, lo + per);
+    // (8 loads in flight per lane: the lanes' segments lie far apart, so each load waits on memory)
+    int64_t own = 0;
+    int64_t i = lo;
+    for (; i + 8 <= hi; i += 8) {
+        int64_t x[8];
+        for (int k = 0; k < 8; ++k) x[k] = bs[i + k];
+        for (int k = 0; k < 8; ++k) own += x[k];
+    }
+    for (; i < hi; ++i) own += bs[i];
+    // inclusive scan of the lanes' totals: within the wave, then over the wave totals
+    int64_t v = own;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t o = shfl_up_i64(v, d);
+        if (lane >= d) v += o;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        int64_t x = lane < kScanSumsBlock / 64 ? wsum[lane] : 0;
+        for (int d = 1; d < kScanSumsBlock / 64; d <<= 1) {
+            const int64_t o = shfl_up_i64(x, d);
+            if (lane >= d) x += o;
+        }
+        if (lane < kScanSumsBlock / 64) wsum[lane] = x;
+    }
+    __syncthreads();
+    int64_t run = (w > 0 ? wsum[w - 1] : 0) + v - own;
+    for (i = lo; i + 8 <= hi; i += 8) {
+        int64_t x[8];
+        for (int k = 0; k < 8; ++k) x[k] = bs[i + k];
+        for (int k = 0; k < 8; ++k) {
+            bs[i + k] = run;
+            run += x[k];
+        }
+    }
+    for (; i < hi; ++i) {
+        const int64_t x = bs[i];
+        bs[i] = run;
+        run += x;
+    }
+    if (t == kScanSumsBlock - 1) *total = wsum[kScanSumsBlock / 64 - 1];
 }
 
 // (sizes: when given, the pair count capped at n_pairs and the tile count sizes[1] come from the
@@ -867,11 +935,11 @@ hipError_t launch_scan_u32(const unsigned* in, int64_t n, int64_t* out, int64_t*
                            hipStream_t stream) {
     const int64_t nblk = (n + kScanBlock - 1) / kScanBlock;
     if (nblk > 0)
-        hipLaunchKernelGGL(k_scan_local, dim3(unsigned(nblk)), dim3(256), 0, stream, in, n, (const int64_t*)nullptr, out,
+        hipLaunchKernelGGL(k_scan_reduce, dim3(unsigned(nblk)), dim3(256), 0, stream, in, n, (const int64_t*)nullptr,
                            block_sum);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, stream, block_sum, nblk, total);  // (nblk 0: total 0)
-    hipLaunchKernelGGL(k_scan_add, dim3(unsigned((n + 1 + 255) / 256)), dim3(256), 0, stream, out, n,
-                       (const int64_t*)nullptr, block_sum, total);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanSumsBlock), 0, stream, block_sum, nblk, total);  // (nblk 0: total 0)
+    hipLaunchKernelGGL(k_scan_apply, dim3(unsigned(n / kScanBlock + 1)), dim3(256), 0, stream, in, n,
+                       (const int64_t*)nullptr, block_sum, total, out);
     return hipGetLastError();
 }
 
@@ -903,11 +971,10 @@ static void enqueue_candidate_space(hipStream_t stream, const LatticeView& lat, 
     hipLaunchKernelGGL(k_pair_setup, dim3(unsigned((n_pairs + 255) / 256)), dim3(256), 0, stream, lat, ct, pairs, n_pairs,
                        n_dev, grid_off, n_grids, pair_grid, d_grids, tet_mass, precision == ADECO_PREC_FP64 ? kMaxSeg : 1,
                        cs.cand, cs.pd);
-    hipLaunchKernelGGL(k_scan_local, dim3(unsigned(nblk)), dim3(256), 0, stream, cs.cand, n_pairs, n_dev, cs.cand_off,
-                       cs.block_sum);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, stream, cs.block_sum, nblk, cs.total);
-    hipLaunchKernelGGL(k_scan_add, dim3(unsigned((n_pairs + 1 + 255) / 256)), dim3(256), 0, stream, cs.cand_off, n_pairs,
-                       n_dev, cs.block_sum, cs.total);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(unsigned(nblk)), dim3(256), 0, stream, cs.cand, n_pairs, n_dev, cs.block_sum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanSumsBlock), 0, stream, cs.block_sum, nblk, cs.total);
+    hipLaunchKernelGGL(k_scan_apply, dim3(unsigned(n_pairs / kScanBlock + 1)), dim3(256), 0, stream, cs.cand, n_pairs,
+                       n_dev, cs.block_sum, cs.total, cs.cand_off);
 }
 
 adeco_status build_candidate_space(hipStream_t stream, const LatticeView& lat, const PackedCorners& ct,
