@@ -246,11 +246,18 @@ __global__ void __launch_bounds__(256) k_bin_map(LatticeView lat, PackedCorners 
             if (cells > kMapCells) {
                 over = true;
             } else {
-                // the fine cells' leaves, each distinct leaf once (fixed slots: registers only)
+                // the fine cells' leaves, each distinct leaf once (fixed slots: registers only).  Slot c
+                // -> (c mod n0, (c / n0) mod n1, c / (n0 n1)) without integer division: for c < 8 and
+                // 1 <= n <= 8, floor(c / n) = (c * ceil(64 / n)) >> 6 exactly (ceil(64 / n) packed in 7
+                // bits per n)
+                constexpr unsigned long long kCeil64Over = 0x102858d2059040ull;
+                const int m0 = int((kCeil64Over >> (7 * (n[0] - 1))) & 127u);
+                const int m1 = int((kCeil64Over >> (7 * (n[1] - 1))) & 127u);
 #pragma unroll
                 for (int c = 0; c < kMapCells; ++c) {
                     if (c < cells) {
-                        const int cx = c % n[0], cy = (c / n[0]) % n[1], cz = c / (n[0] * n[1]);
+                        const int q0 = (c * m0) >> 6, q1 = (q0 * m1) >> 6;
+                        const int cx = c - q0 * n[0], cy = q0 - q1 * n[1], cz = q1;
                         leaf[c] = lm.map[(lo[0] + cx) + int64_t(lm.F) * ((lo[1] + cy) + int64_t(lm.F) * (lo[2] + cz))];
                     }
                 }
