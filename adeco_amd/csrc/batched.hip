@@ -192,7 +192,8 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
                                                     const GridDesc* __restrict__ grids, double tet_mass, int max_seg,
                                                     unsigned* __restrict__ cand, PairData* __restrict__ pd) {
     // the 128-byte lines are assembled in LDS (rows of 17 words: the odd stride spreads a wave's
-    // row writes over the banks) and stored as whole contiguous rows of the block
+    // row writes over the banks) and stored as whole contiguous rows of the block, non-temporal
+    // (the 16 GB of lines of a large launch pass through no cache usefully)
     constexpr int kWords = int(sizeof(PairData) / 8);
     __shared__ unsigned long long rows[256 * (kWords + 1)];
     // the pair count: n_pairs, or the device's count capped at n_pairs (a uniform exit per block)
@@ -209,7 +210,8 @@ __global__ void __launch_bounds__(256) k_pair_setup(LatticeView lat, PackedCorne
     __syncthreads();
     const int rows_here = int(min(int64_t(256), n_pairs - base));
     unsigned long long* dst = reinterpret_cast<unsigned long long*>(pd + base);
-    for (int k = threadIdx.x; k < rows_here * kWords; k += 256) dst[k] = rows[(k / kWords) * (kWords + 1) + k % kWords];
+    for (int k = threadIdx.x; k < rows_here * kWords; k += 256)
+        __builtin_nontemporal_store(rows[(k / kWords) * (kWords + 1) + k % kWords], dst + k);
 }
 
 // grid index of every pair of grid-major pairs below cap: one workgroup per grid writes its range
