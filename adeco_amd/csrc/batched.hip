@@ -18,12 +18,13 @@
 //                       tet's 4 face planes at the extreme corners (a relative margin of 1e-12: an
 //                       uncertain segment is clipped, never skipped or filled).  Segments inside the
 //                       tet and cells holding the whole tet are deposited at once; partial ones go
-//                       to one of 22 per-wave LDS queues keyed by their clip work class (number of
-//                       x/y faces that cut the tet once the column axis - for a single cell the axis
-//                       with the most cutting faces - is taken as the closed-form z axis, and how
-//                       many vertices lie inside each of the clip tree's first two faces).  A queue
-//                       that reaches 64 entries is clipped by the whole wave, so the lanes of one
-//                       clip follow nearly the same path through the nested clip.
+//                       to one of 22 per-wave LDS queues (11 in a server step) keyed by their clip
+//                       work class (number of x/y faces that cut the tet once the column axis - for
+//                       a single cell the axis with the most cutting faces - is taken as the
+//                       closed-form z axis, and how many vertices lie inside each of the clip tree's
+//                       first two faces).  A queue that reaches 64 entries is clipped by the whole
+//                       wave, so the lanes of one clip follow nearly the same path through the
+//                       nested clip.
 // One x/y clip serves all cells of a segment: every leaf piece is integrated over the z planes
 // between the cells in closed form, each plane once for the two cells it separates, into per-lane
 // LDS sums.  The clip uses the signed tree (a cut leaving 3 vertices inside gives the tet minus the
@@ -83,10 +84,9 @@ constexpr int kBuckets = 23;       // 22 clip work classes + the exact-recompute
 constexpr int kExactQueue = 22;
 // launches with fewer tiles than this per resident wave drain their queues in merged passes
 constexpr int64_t kMergeDrainTiles = 16;
-// how a wave drains its queues once the work items run out: a pass per queue (a large launch,
-// where the drain is a small part of the work), passes filled from the classes of one group (a
-// step of the request server, which drains every step), or from all the classes (a small launch)
-constexpr int kDrainPerClass = 0, kDrainGroup = 1, kDrainAll = 2;
+// how a wave drains its queues once the work items run out: a pass per queue (a large launch or a
+// step of the request server), or passes filled from all the classes (a small launch)
+constexpr int kDrainPerClass = 0, kDrainAll = 1;
 
 // Per-wavefront LDS: the bucketed partial-candidate queues (64 entries each), the entries a tile
 // could not place because their queue was full (at most one per lane, written back once that queue
@@ -423,10 +423,13 @@ __device__ __forceinline__ unsigned separated_cells(const V3<double> t[4], int l
 // Clip work class of a partial cell: the axis with the most faces cutting the tet is integrated in
 // closed form (ties -> z); the class counts the remaining x/y faces that cut (from the cell-local
 // AABB), whether one face of each axis cuts or both of one, and how many vertices lie inside the
-// first two faces of the clip tree.  The class only groups cells of similar clip work (lanes of a
-// pass then take the same branches of the clip tree more often); the clip itself decides every
-// face again.
+// first two faces of the clip tree (22 classes; kFine false: 11 classes by the number of faces and
+// the first face's count, for the request server's steps, which drain all their queues every step,
+// so that fewer classes leave fewer partly filled drain passes).  The class only groups cells of
+// similar clip work (lanes of a pass then take the same branches of the clip tree more often); the
+// clip itself decides every face again.
 // A column segment (force >= 0) keeps its axis: force.
+template <bool kFine>
 __device__ __forceinline__ int work_class(const V3<double> s[4], const double mn[3], const double mx[3], int force,
                                           int& axis) {
     const bool lx = mn[0] < 0.0, hx = mx[0] > 1.0, ly = mn[1] < 0.0, hy = mx[1] > 1.0, lz = mn[2] < 0.0, hz = mx[2] > 1.0;
@@ -463,17 +466,12 @@ __device__ __forceinline__ int work_class(const V3<double> s[4], const double mn
     // vertices inside the first and second face (1-3 each; with both X faces cutting, every vertex
     // is inside one of them, so n1 + n2 >= 4)
     const int t1 = n1 <= 1 ? 0 : (n1 == 2 ? 1 : 2), t2 = n2 <= 1 ? 0 : (n2 == 2 ? 1 : 2);
+    // coarse classes (2-10): the number of cutting faces and the first face's count only
+    if (!kFine) return 3 * nact - 4 + t1;
     if (xy) return 2 + 3 * t1 + t2;                                           // 2-10
     if (nact == 2) return 11;                                                 // both X faces
     if (nact == 3) return 12 + (t1 == 0 ? 0 : (t1 == 1 ? (t2 <= 1 ? 1 : 2) : 3 + t2));  // 12-17
     return 18 + (t1 <= 1 ? 0 : 1 + t2);                                       // 18-21
-}
-
-// Work classes whose clips share the tree's shape (no or one cutting face; a corner of the
-// column's cross-section; both faces of one axis; three faces; four faces): a drain pass fills its
-// lanes from the classes of one group.
-__device__ __forceinline__ int class_group(int c) {
-    return c <= 1 ? 0 : (c <= 10 ? 1 : (c == 11 ? 2 : (c <= 17 ? 3 : 4)));
 }
 
 // n / d for n < 2^30 and 1 <= d < 2^30 without the integer division sequence: the FP64 reciprocal
@@ -532,6 +530,7 @@ __device__ __forceinline__ TileHead load_head(const Sources& srcs, const int2* _
 // Classifies the 64 candidates of tile h.t of `src` (slot h.slot among the launch's sources):
 // segments inside the tet and tets inside a single cell are deposited, partial candidates are
 // appended to the wave's queues.
+template <bool kFine>
 __device__ __forceinline__ void classify_tile(const TileSource& src, const TileHead& h, WaveLds& L, int lane,
                                               unsigned long long& n_partial) {
     const int slot = h.slot, p0 = h.p0, np = h.np;
@@ -607,7 +606,7 @@ __device__ __forceinline__ void classify_tile(const TileSource& src, const TileH
                     // a segment keeps the pair's axis (its length is measured along it); a single
                     // cell takes the axis with the most cutting faces
                     int axis;
-                    key = work_class(s, mn, mx, col && ea > 1 ? pa : -1, axis);
+                    key = work_class<kFine>(s, mn, mx, col && ea > 1 ? pa : -1, axis);
                     cell = int(unsigned(i) | (unsigned(j) << 10) | (unsigned(k) << 20) | (unsigned(axis) << 30));
                 }
             }
@@ -728,7 +727,7 @@ struct ManySources {
 // directly (one source) or through `entries`.  The wave pops `chunk` items per work-queue access;
 // a queue reaching 64 entries is clipped at once; when the items run out the queues are drained.
 // Every wave leaves the loop once the counter passes n_items and its queues are empty.
-template <int kPrec, int kDrain, class Sources>
+template <int kPrec, int kDrain, bool kFine, class Sources>
 __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __restrict__ entries, int64_t n_items,
                                           int chunk, unsigned* __restrict__ queue_counter,
                                           unsigned long long* __restrict__ counters) {
@@ -780,16 +779,15 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
         if (ready) {
             const int b = (ready >> kExactQueue) & 1ull ? kExactQueue : __builtin_ctzll(ready);
             const int cnt = __builtin_amdgcn_readfirstlane(__shfl(qv, b, 64));
-            // this pass's entry of each lane: from queue b, or when a server step or a small launch
-            // drains with no full queue, up to 64 entries across the work classes of b's group (of
-            // all groups in a small launch; the classes only group similar work, the clip does not
-            // depend on them), from the top of each queue, lowest class first.  Either way a pass
-            // takes >= 1 entry, so the drain ends.
+            // this pass's entry of each lane: from queue b, or when draining a small launch with no
+            // full queue, up to 64 entries across all the work classes (they only group similar
+            // work; the clip does not depend on them), from the top of each queue, lowest class
+            // first.  Either way a pass takes >= 1 entry, so the drain ends.
             int2 e = make_int2(0, 0);
             int slot = 0;
             bool have = false;
-            if (kDrain != kDrainPerClass && draining && b != kExactQueue && cnt < kWaveTile) {
-                const bool merge = lane < kExactQueue && (kDrain == kDrainAll || class_group(lane) == class_group(b));
+            if (kDrain == kDrainAll && draining && b != kExactQueue && cnt < kWaveTile) {
+                const bool merge = lane < kExactQueue;
                 const int qc = merge ? qv : 0;
                 int pre = qc;
                 for (int d = 1; d < 32; d <<= 1) {
@@ -857,7 +855,7 @@ __device__ __forceinline__ void wave_loop(const Sources& srcs, const int2* __res
         const TileHead cur = head;
         ++it;
         if (it < it_end) head = load_head(srcs, entries, it, lane);
-        classify_tile(srcs.get(cur.slot), cur, L, lane, n_partial);
+        classify_tile<kFine>(srcs.get(cur.slot), cur, L, lane, n_partial);
     }
     if (lane == 0 && n_partial) atomicAdd(counters + 1, n_partial);
     if (fb) atomicAdd(counters, fb);
@@ -870,7 +868,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_deposit_waves(TileSource src, int
         src.n_cand = src.dev_sizes[0];
         src.n_tiles = src.dev_sizes[1];
     }
-    wave_loop<kPrec, kMergeDrain ? kDrainAll : kDrainPerClass>(OneSource{src}, nullptr, src.n_tiles, chunk,
+    wave_loop<kPrec, kMergeDrain ? kDrainAll : kDrainPerClass, true>(OneSource{src}, nullptr, src.n_tiles, chunk,
                                                                 queue_counter, counters);
 }
 
@@ -879,7 +877,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_deposit_step(const TileSource* __
                                                          const int2* __restrict__ entries, int n_entries, int chunk,
                                                          unsigned* __restrict__ queue_counter,
                                                          unsigned long long* __restrict__ counters) {
-    wave_loop<kPrec, kMergeDrain ? kDrainAll : kDrainGroup>(ManySources{sources}, entries, n_entries, chunk,
+    wave_loop<kPrec, kMergeDrain ? kDrainAll : kDrainPerClass, false>(ManySources{sources}, entries, n_entries, chunk,
                                                              queue_counter, counters);
 }
 
